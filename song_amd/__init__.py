@@ -1,0 +1,143 @@
+"""song_amd: MI355X (gfx950) batched graph search behind a C ABI.
+
+The search runs only in the HIP library ``song_amd/lib/libsong_hip.so``
+(declared in include/song_amd.h); this module is a thin ctypes binding over
+that ABI.  Importing it does not need a GPU; calling a search without the
+library or without a GPU raises ``SongError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .datasets import METRIC_COS, METRIC_IP, METRIC_L2, METRIC_NAMES  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsong_hip.so")
+
+MODE_AUTO, MODE_EXACT, MODE_FP32 = 0, 1, 2
+FLAG_VISITED_OVF, FLAG_QUEUE_OVF, FLAG_STEP_CAP = 1, 2, 4
+
+_lib = None
+
+
+class SongError(RuntimeError):
+    pass
+
+
+class SearchOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("slack", ctypes.c_int), ("hash_log2", ctypes.c_int),
+                ("max_steps", ctypes.c_int), ("retry", ctypes.c_int)]
+
+
+def lib():
+    """Load the HIP library (raises SongError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SongError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        # One HIP runtime per process: torch ships its own libamdhip64 (SONAME
+        # libamdhip64.so.7).  Loading torch first lets this library's NEEDED entry bind
+        # to that copy instead of starting a second runtime from /opt/rocm.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        L.song_last_error.restype = ctypes.c_char_p
+        L.song_search_opts_default.argtypes = [ctypes.POINTER(SearchOpts)]
+        L.song_index_create.argtypes = [vp, u64, i32, vp, i32, i32, ctypes.POINTER(vp)]
+        L.song_index_destroy.argtypes = [vp]
+        L.song_index_integer_valued.argtypes = [vp]
+        L.song_search.argtypes = [vp, vp, u64, i32, i32, vp, vp, vp, ctypes.POINTER(SearchOpts)]
+        L.song_search_device.argtypes = [vp, vp, u64, i32, i32, vp, vp, vp, vp, i32, vp, vp,
+                                         ctypes.POINTER(SearchOpts), vp]
+        L.song_astar_multi_start_search_batch.argtypes = [vp, u64, i32, vp, vp, vp, i32, u64, i32, i32]
+        _lib = L
+    return _lib
+
+
+EXPORTED_SYMBOLS = (
+    "song_search_opts_default", "song_index_create", "song_index_destroy", "song_index_integer_valued",
+    "song_search", "song_search_device", "song_astar_multi_start_search_batch", "song_last_error",
+)
+
+
+def _check(rc):
+    if rc != 0:
+        raise SongError(f"song error {rc}: {lib().song_last_error().decode(errors='replace')}")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def metric_code(metric):
+    return METRIC_NAMES[metric] if isinstance(metric, str) else int(metric)
+
+
+def make_opts(mode=MODE_AUTO, slack=32, hash_log2=0, max_steps=0, retry=1):
+    o = SearchOpts()
+    lib().song_search_opts_default(ctypes.byref(o))
+    o.mode, o.slack, o.hash_log2, o.max_steps, o.retry = mode, slack, hash_log2, max_steps, retry
+    return o
+
+
+class Index:
+    """An uploaded (data, graph) pair.  data: float32 [n, dim]; graph: uint64 [n, 2**shift]."""
+
+    def __init__(self, data, graph, metric, shift=5):
+        self.data = np.ascontiguousarray(data, dtype=np.float32)
+        g = np.ascontiguousarray(graph, dtype=np.uint64)
+        self.n, self.dim = self.data.shape
+        self.metric = metric_code(metric)
+        h = ctypes.c_void_p()
+        _check(lib().song_index_create(_ptr(self.data), self.n, self.dim, _ptr(g), shift, self.metric,
+                                       ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().song_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def integer_valued(self):
+        return bool(lib().song_index_integer_valued(self._h))
+
+    def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=32, hash_log2=0, retry=1):
+        """Host buffers in and out: ids int64 [nq, k] (-1 padding), dists float32, flags uint32."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        k = pq if k is None else min(k, pq)
+        ids = np.empty((nq, k), dtype=np.uint64)
+        ds = np.empty((nq, k), dtype=np.float32)
+        fl = np.empty(nq, dtype=np.uint32)
+        o = make_opts(mode, slack, hash_log2, 0, retry)
+        _check(lib().song_search(self._h, _ptr(q), nq, pq, k, _ptr(ids), _ptr(ds), _ptr(fl), ctypes.byref(o)))
+        out = ids.astype(np.int64)
+        out[ids == np.uint64(0xFFFFFFFFFFFFFFFF)] = -1
+        return out, ds, fl
+
+    def search_device(self, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
+                      mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, stream=None):
+        """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k]."""
+        import torch
+
+        nq = queries.shape[0]
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        o = make_opts(mode, slack, hash_log2, 0, retry)
+        _check(lib().song_search_device(self._h, dp(queries), nq, pq, k, dp(pq_arr), dp(k_arr), dp(ids), dp(dists),
+                                        ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
+                                        ctypes.c_void_p(stream)))

@@ -1,0 +1,479 @@
+// Best-first graph search on gfx950: one query per 64-lane wavefront.
+//
+// Semantics follow the reference CPU search FixedDegreeGraph::astar_multi_start_search
+// (graph.h:169-216): start at the entry vertex, repeatedly expand the (dist, id)-smallest
+// unexpanded candidate, stop when the k-best expanded list is full and its largest distance
+// is strictly below the next candidate's (graph.h:188), mark every discovered neighbour
+// visited (graph.h:198-205), return the expanded list ascending by (dist, id).
+//
+// The CPU's unbounded queue is replaced by a bounded sorted array that only ever drops
+// entries that can never be expanded: an entry c is dropped when the pq-best expanded list
+// is full with largest distance < c.d, or when the queue already holds pq entries with
+// distance < c.d.  Entries tied with the pq-th distance are kept (up to `slack`), so the
+// expansion order -- and therefore the output -- equals the CPU's.
+//
+// Per-wave LDS layout (bytes, in order):
+//   visited  : H x u32 open-addressing set of (id + 1), linear probing, no deletion
+//   queue    : QS*64 keys, sorted ascending, live range [h, h + nq)
+//   best     : QS*64 keys, sorted ascending, first nbest live (pq-best expanded list)
+//   ckey     : 64 keys   candidate keys of the current expansion
+//   cid      : 64 u32    candidate ids of the current expansion
+//   cflag    : 64 u32    candidate needs sequential re-evaluation (exact mode)
+//   qv       : ld f32    the query, zero-padded to ld
+//
+// Keys order by (distance, id) lexicographically:
+//   KeyF (fp32 mode):  one u64 = (orderable float bits << 32) | id
+//   KeyD (exact mode): orderable double bits + u32 id
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace song {
+
+enum { M_L2 = 0, M_IP = 1, M_COS = 2 };
+
+enum : uint32_t { F_VIS_OVF = 1u, F_Q_OVF = 2u, F_STEP_CAP = 4u };
+
+struct SearchParams {
+    const float *data;      // n x ld
+    const uint32_t *graph;  // n x gstride, [deg, ids...]
+    const float *norms;     // cosine: per-vertex squared norm (fmaf chain), else null
+    const float *queries;   // nq x dim
+    const uint32_t *entry;  // per-query start vertex or null (vertex 0)
+    const int *pq_arr;      // per-query pq or null
+    const int *k_arr;       // per-query k or null
+    const uint32_t *qidx;   // query index remap (retry pass) or null
+    uint32_t *out_ids;      // nq x out_ld
+    float *out_dist;        // nq x out_ld or null
+    uint32_t *out_flags;    // nq or null
+    uint32_t *out_stats;    // nq x 2 or null
+    uint64_t n;
+    int dim, ld, gstride;
+    int nq, pq, k, out_ld;
+    int hlog, slack, max_steps;
+};
+
+// ---------------------------------------------------------------- keys
+
+__device__ __forceinline__ uint32_t ord32(float f) {
+    uint32_t u = __float_as_uint(f);
+    if (f == 0.0f) u = 0u;            // -0 and +0 compare equal on the CPU
+    if (f != f) u = 0x7f800000u;      // NaN (zero-norm cosine) sorts as +inf
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord32(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ uint64_t ord64(double d) {
+    uint64_t u = (uint64_t)__double_as_longlong(d);
+    if (d == 0.0) u = 0ull;
+    if (d != d) u = 0x7ff0000000000000ull;
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unord64(uint64_t o) {
+    uint64_t u = (o & 0x8000000000000000ull) ? (o & 0x7fffffffffffffffull) : ~o;
+    return __longlong_as_double((long long)u);
+}
+
+__device__ __forceinline__ uint32_t rl32(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int lane) {
+    return ((uint64_t)rl32((uint32_t)(v >> 32), lane) << 32) | rl32((uint32_t)v, lane);
+}
+
+struct KeyF {
+    uint64_t v;
+    __device__ static KeyF make_f(float d, uint32_t id) { return KeyF{((uint64_t)ord32(d) << 32) | id}; }
+    __device__ static KeyF inf() { return KeyF{~0ull}; }
+    __device__ uint32_t id() const { return (uint32_t)v; }
+    __device__ uint64_t dk() const { return v >> 32; }
+    __device__ float dist() const { return unord32((uint32_t)(v >> 32)); }
+    __device__ KeyF rl(int lane) const { return KeyF{rl64(v, lane)}; }
+};
+__device__ __forceinline__ bool lt(const KeyF &a, const KeyF &b) { return a.v < b.v; }
+
+struct KeyD {
+    uint64_t d;
+    uint32_t i, pad;
+    __device__ static KeyD make_d(double dist, uint32_t id) { return KeyD{ord64(dist), id, 0u}; }
+    __device__ static KeyD inf() { return KeyD{~0ull, ~0u, 0u}; }
+    __device__ uint32_t id() const { return i; }
+    __device__ uint64_t dk() const { return d; }
+    __device__ float dist() const { return (float)unord64(d); }
+    __device__ KeyD rl(int lane) const { return KeyD{rl64(d, lane), rl32(i, lane), 0u}; }
+};
+__device__ __forceinline__ bool lt(const KeyD &a, const KeyD &b) {
+    return a.d < b.d || (a.d == b.d && a.i < b.i);
+}
+
+template <bool EXACT> struct KeySel { using T = KeyF; };
+template <> struct KeySel<true> { using T = KeyD; };
+
+// ---------------------------------------------------------------- wave helpers
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// lowest set bit exponent of a nonzero float (value = odd * 2^e); INT_MAX for 0
+__device__ __forceinline__ int lsb_exp(float p) {
+    uint32_t u = __float_as_uint(p) & 0x7fffffffu;
+    if (u == 0u) return 0x3fffffff;
+    int e = (int)(u >> 23);
+    uint32_t m = u & 0x7fffffu;
+    if (e) m |= 0x800000u;
+    else e = 1;
+    return e - 150 + __builtin_ctz(m);
+}
+
+// visited set: insert id, return true if it was absent.  Lanes inserting the same id
+// race through compare-and-swap; exactly one of them sees the slot empty.
+__device__ __forceinline__ bool vis_insert(uint32_t *vis, uint32_t mask, int hlog, uint32_t id) {
+    uint32_t key = id + 1u;
+    uint32_t s = (id * 0x9E3779B1u) >> (32 - hlog);
+    for (;;) {
+        uint32_t old = atomicCAS(&vis[s], 0u, key);
+        if (old == 0u) return true;
+        if (old == key) return false;
+        s = (s + 1u) & mask;
+    }
+}
+
+// ---------------------------------------------------------------- distances
+
+// Final distance from the accumulated dot/L2 sum, per reference data.h:28-62.
+template <int METRIC>
+__device__ __forceinline__ double finish_exact(double s, float lena, float lenv) {
+    if (METRIC == M_L2) return s;
+    if (METRIC == M_IP) return -s;
+    int sign = s < 0 ? 1 : -1;
+    return (double)sign * (s * s / (double)lena / (double)lenv);
+}
+template <int METRIC>
+__device__ __forceinline__ float finish_f32(float s, float lena, float lenv) {
+    if (METRIC == M_L2) return s;
+    if (METRIC == M_IP) return -s;
+    float sign = s < 0 ? 1.0f : -1.0f;
+    return sign * (s * s / lena / lenv);
+}
+
+// One lane, sequential over the dimensions in the CPU's order (exact fallback).
+template <int METRIC>
+__device__ double seq_sum(const float *row, const float *qv, int dim) {
+    double s = 0.0;
+    for (int i = 0; i < dim; ++i) {
+        float x = row[i], q = qv[i];
+        float p;
+        if (METRIC == M_L2) {
+            float d = x - q;
+            p = d * d;
+        } else {
+            p = x * q;
+        }
+        s += (double)p;
+    }
+    return s;
+}
+
+// Distances of the m candidates cid[0..m) -> ckey[0..m).  SUB lanes share one row.
+template <int METRIC, bool EXACT, int SUB, class Key>
+__device__ void eval_candidates(const SearchParams &P, int m, const uint32_t *cid, Key *ckey, uint32_t *cflag,
+                                const float *qv, float lenv) {
+    const int lane = threadIdx.x;
+    constexpr int G = 64 / SUB;
+    const int g = lane / SUB, sl = lane % SUB;
+    const int nv4 = P.ld >> 2;
+    const float4 *q4 = reinterpret_cast<const float4 *>(qv);
+    bool need_seq = false;
+    for (int base = 0; base < m; base += G) {
+        const int c = base + g;
+        const bool valid = c < m;
+        const uint32_t id = valid ? cid[c] : 0u;
+        const float4 *row = reinterpret_cast<const float4 *>(P.data + (size_t)id * (size_t)P.ld);
+        if constexpr (!EXACT) {
+            float acc = 0.0f;
+            if (valid) {
+                for (int j = sl; j < nv4; j += SUB) {
+                    float4 x = row[j], q = q4[j];
+                    if (METRIC == M_L2) {
+                        float a = x.x - q.x, b = x.y - q.y, cc = x.z - q.z, d = x.w - q.w;
+                        acc += a * a + b * b + cc * cc + d * d;
+                    } else {
+                        acc += x.x * q.x + x.y * q.y + x.z * q.z + x.w * q.w;
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = SUB / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            if (valid && sl == 0) {
+                float lena = (METRIC == M_COS) ? P.norms[id] : 0.0f;
+                ckey[c] = Key::make_f(finish_f32<METRIC>(acc, lena, lenv), id);
+            }
+        } else {
+            // float products (rounded), double accumulation; exact in any order when all
+            // products are multiples of 2^mn and their absolute sum is below 2^(mn+53).
+            double acc = 0.0, sab = 0.0;
+            int mn = 0x3fffffff;
+            if (valid) {
+                for (int j = sl; j < nv4; j += SUB) {
+                    float4 x = row[j], q = q4[j];
+                    float p0, p1, p2, p3;
+                    if (METRIC == M_L2) {
+                        float a = x.x - q.x, b = x.y - q.y, cc = x.z - q.z, d = x.w - q.w;
+                        p0 = a * a; p1 = b * b; p2 = cc * cc; p3 = d * d;
+                    } else {
+                        p0 = x.x * q.x; p1 = x.y * q.y; p2 = x.z * q.z; p3 = x.w * q.w;
+                    }
+                    acc += (double)p0; acc += (double)p1; acc += (double)p2; acc += (double)p3;
+                    sab += (double)fabsf(p0); sab += (double)fabsf(p1);
+                    sab += (double)fabsf(p2); sab += (double)fabsf(p3);
+                    mn = min(mn, min(min(lsb_exp(p0), lsb_exp(p1)), min(lsb_exp(p2), lsb_exp(p3))));
+                }
+            }
+#pragma unroll
+            for (int o = SUB / 2; o > 0; o >>= 1) {
+                acc += __shfl_xor(acc, o);
+                sab += __shfl_xor(sab, o);
+                mn = min(mn, __shfl_xor(mn, o));
+            }
+            if (valid && sl == 0) {
+                bool exact = (sab == 0.0) || (mn < 0x3fffffff && sab < ldexp(1.0, mn + 53));
+                float lena = (METRIC == M_COS) ? P.norms[id] : 0.0f;
+                ckey[c] = Key::make_d(finish_exact<METRIC>(acc, lena, lenv), id);
+                cflag[c] = exact ? 0u : 1u;
+            }
+            need_seq = true;
+        }
+    }
+    if constexpr (EXACT) {
+      if (need_seq) {
+        wave_sync();
+        // sequential re-evaluation, one lane per flagged candidate
+        if (lane < m && cflag[lane]) {
+            const uint32_t id = cid[lane];
+            double s = seq_sum<METRIC>(P.data + (size_t)id * (size_t)P.ld, qv, P.dim);
+            float lena = (METRIC == M_COS) ? P.norms[id] : 0.0f;
+            ckey[lane] = Key::make_d(finish_exact<METRIC>(s, lena, lenv), id);
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------- the search kernel
+
+template <int METRIC, bool EXACT, int SUB, int QS>
+__global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
+    using Key = typename KeySel<EXACT>::T;
+    constexpr int CAP = QS * 64;
+    extern __shared__ __align__(16) unsigned char smem[];
+
+    const int lane = threadIdx.x;
+    const int slot_q = (int)blockIdx.x;
+    const int qi = P.qidx ? (int)P.qidx[slot_q] : slot_q;
+    const int pq = P.pq_arr ? P.pq_arr[qi] : P.pq;
+    const int k = P.k_arr ? min(P.k_arr[qi], pq) : min(P.k, pq);
+    const int capq = pq + P.slack;
+    const uint32_t H = 1u << P.hlog;
+    const uint32_t hmask = H - 1u;
+    const uint32_t vis_limit = (H >> 2) * 3u;
+
+    uint32_t *vis = reinterpret_cast<uint32_t *>(smem);
+    Key *Q = reinterpret_cast<Key *>(smem + (size_t)H * 4u);
+    Key *B = Q + CAP;
+    Key *ckey = B + CAP;
+    uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + 64);
+    uint32_t *cflag = cid + 64;
+    float *qv = reinterpret_cast<float *>(cflag + 64);
+
+    for (uint32_t i = lane; i < H; i += 64) vis[i] = 0u;
+    for (int i = lane; i < P.ld; i += 64) qv[i] = (i < P.dim) ? P.queries[(size_t)qi * P.dim + i] : 0.0f;
+    wave_sync();
+
+    float lenv = 0.0f;
+    if (METRIC == M_COS) {
+        // query norm in the CPU's order: lenv = fmaf(v[i], v[i], lenv) (data.h:55 under FMA contraction)
+        if (lane == 0)
+            for (int i = 0; i < P.dim; ++i) lenv = fmaf(qv[i], qv[i], lenv);
+        lenv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(lenv)));
+    }
+
+    uint32_t flags = 0u;
+    int steps = 0;
+    uint32_t nvis = 0u;
+    int h = 0, nq = 0, nbest = 0;
+
+    // start vertex (graph.h:175-181)
+    const uint32_t start = P.entry ? P.entry[qi] : 0u;
+    if (P.n > 0) {
+        if (lane == 0) {
+            vis_insert(vis, hmask, P.hlog, start);
+            cid[0] = start;
+        }
+        nvis = 1u;
+        wave_sync();
+        eval_candidates<METRIC, EXACT, SUB, Key>(P, 1, cid, ckey, cflag, qv, lenv);
+        wave_sync();
+        if (lane == 0) Q[0] = ckey[0];
+        nq = 1;
+        wave_sync();
+    }
+
+    while (nq > 0) {
+        if (steps >= P.max_steps) {
+            flags |= F_STEP_CAP;
+            break;
+        }
+        const Key now = Q[h];
+        // stop rule, graph.h:188: |best| == pq and best.max.d < now.d (distance only, strict)
+        if (nbest == pq && B[pq - 1].dk() < now.dk()) break;
+        ++steps;
+        ++h;
+        --nq;
+
+        // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195)
+        {
+            Key t[QS];
+            int below = 0;
+#pragma unroll
+            for (int s = 0; s < QS; ++s) {
+                const int i = lane + 64 * s;
+                t[s] = (i < nbest) ? B[i] : Key::inf();
+                below += __popcll(ballot(i < nbest && lt(t[s], now)));
+            }
+            if (below < pq) {
+                wave_sync();
+#pragma unroll
+                for (int s = 0; s < QS; ++s) {
+                    const int i = lane + 64 * s;
+                    if (i >= below && i < nbest && i + 1 < pq) B[i + 1] = t[s];
+                }
+                if (lane == 0) B[below] = now;
+                nbest = min(nbest + 1, pq);
+                wave_sync();
+            }
+        }
+
+        // neighbours of `now` in row order (graph.h:196-205)
+        const uint32_t *row = P.graph + (size_t)now.id() * (size_t)P.gstride;
+        const uint32_t r = (lane < P.gstride) ? row[lane] : 0u;
+        const int deg = (int)rl32(r, 0);
+        const bool fresh = (lane >= 1 && lane <= deg) ? vis_insert(vis, hmask, P.hlog, r) : false;
+        const uint64_t fm = ballot(fresh);
+        const int m = __popcll(fm);
+        if (fresh) cid[lanes_below(fm)] = r;
+        nvis += (uint32_t)m;
+        if (nvis > vis_limit) {
+            flags |= F_VIS_OVF;
+            break;
+        }
+        if (m == 0) continue;
+        wave_sync();
+        eval_candidates<METRIC, EXACT, SUB, Key>(P, m, cid, ckey, cflag, qv, lenv);
+        wave_sync();
+
+        // ---- merge the candidates into the queue
+        uint64_t cut = ~0ull;  // drop anything with distance key > cut
+        if (nbest == pq) cut = B[pq - 1].dk();
+        if (nq >= pq) cut = min(cut, Q[h + pq - 1].dk());
+        const Key mine = (lane < m) ? ckey[lane] : Key::inf();
+        const bool surv = lane < m && mine.dk() <= cut;
+        const uint64_t sm = ballot(surv);
+        const int ms = __popcll(sm);
+        if (ms == 0) continue;
+
+        // position of each survivor: rank among survivors + count of queue entries below it
+        int pos = 0;
+        {
+            uint64_t bits = sm;
+            while (bits) {
+                const int j = __builtin_ctzll(bits);
+                bits &= bits - 1;
+                const Key o = mine.rl(j);
+                pos += lt(o, mine) ? 1 : 0;
+            }
+            if (surv) {
+                int lo = 0, hi = nq;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (lt(Q[h + mid], mine)) lo = mid + 1;
+                    else hi = mid;
+                }
+                pos += lo;
+            }
+        }
+        Key old[QS];
+        int dst[QS];
+#pragma unroll
+        for (int s = 0; s < QS; ++s) {
+            const int i = lane + 64 * s;
+            old[s] = (i < nq) ? Q[h + i] : Key::inf();
+            dst[s] = i;
+        }
+        {
+            uint64_t bits = sm;
+            while (bits) {
+                const int j = __builtin_ctzll(bits);
+                bits &= bits - 1;
+                const Key o = mine.rl(j);
+#pragma unroll
+                for (int s = 0; s < QS; ++s) dst[s] += lt(o, old[s]) ? 1 : 0;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < QS; ++s) {
+            const int i = lane + 64 * s;
+            if (i < nq) Q[dst[s]] = old[s];
+        }
+        if (surv) Q[pos] = mine;
+        h = 0;
+        nq += ms;
+        wave_sync();
+
+        // truncate: keep entries with distance key <= cut2
+        uint64_t cut2 = (nbest == pq) ? B[pq - 1].dk() : ~0ull;
+        if (nq > pq) cut2 = min(cut2, Q[pq - 1].dk());
+        int keep = 0;
+#pragma unroll
+        for (int s = 0; s < QS; ++s) {
+            const int i = lane + 64 * s;
+            const bool kp = i < nq && Q[i].dk() <= cut2;
+            keep += __popcll(ballot(kp));
+        }
+        nq = keep;
+        if (nq > capq) {
+            flags |= F_Q_OVF;
+            break;
+        }
+    }
+
+    // output: first k of the ascending list (graph.h:209-215)
+    const int cnt = min(k, nbest);
+    const size_t ob = (size_t)qi * (size_t)P.out_ld;
+    for (int i = lane; i < k; i += 64) {
+        const bool ok = i < cnt;
+        const Key e = ok ? B[i] : Key::inf();
+        P.out_ids[ob + i] = ok ? e.id() : 0xFFFFFFFFu;
+        if (P.out_dist) P.out_dist[ob + i] = ok ? e.dist() : __int_as_float(0x7f800000);
+    }
+    if (lane == 0) {
+        if (P.out_flags) P.out_flags[qi] = flags;
+        if (P.out_stats) {
+            P.out_stats[2 * (size_t)qi] = (uint32_t)steps;
+            P.out_stats[2 * (size_t)qi + 1] = nvis;
+        }
+    }
+}
+
+template <int METRIC, bool EXACT, int QS>
+constexpr size_t search_lds_bytes(int hlog, int ld) {
+    using Key = typename KeySel<EXACT>::T;
+    return ((size_t)4u << hlog) + 2u * QS * 64u * sizeof(Key) + 64u * sizeof(Key) + 64u * 8u + (size_t)ld * 4u;
+}
+
+}  // namespace song

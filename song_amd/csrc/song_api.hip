@@ -1,0 +1,432 @@
+// C ABI of the graph search (include/song_amd.h): index upload, launch planning,
+// overflow re-runs.  Everything that touches the search runs on the GPU; there is no
+// CPU search path in this library.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/song_amd.h"
+#include "search_inst.hpp"
+
+using namespace song;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(SONG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct song_index {
+    int device = 0;
+    uint64_t n = 0;
+    int dim = 0, ld = 0, metric = 0, shift = 0, gstride = 0;
+    float *d_data = nullptr;
+    uint32_t *d_graph = nullptr;
+    float *d_norms = nullptr;
+    int integer_valued = 0;
+    float vmin = 0.f, vmax = 0.f;
+    size_t max_lds = 65536;
+};
+
+extern "C" const char *song_last_error(void) { return g_err.c_str(); }
+
+extern "C" void song_search_opts_default(song_search_opts *o) {
+    o->mode = SONG_MODE_AUTO;
+    o->slack = 32;
+    o->hash_log2 = 0;
+    o->max_steps = 0;
+    o->retry = 1;
+}
+
+// squared norm per base row with one sequential fmaf chain (data.h:54 as the reference
+// build contracts it), the `lena` of the cosine distance
+__global__ void norms_kernel(const float *data, uint64_t n, int dim, int ld, float *out) {
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float *r = data + v * (uint64_t)ld;
+    float s = 0.0f;
+    for (int i = 0; i < dim; ++i) s = fmaf(r[i], r[i], s);
+    out[v] = s;
+}
+
+// integer-valued test and range of a float array: out[0] = 1 if any non-integer,
+// out[1] / out[2] = min / max as orderable bits
+__global__ void range_kernel(const float *x, uint64_t count, unsigned *out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned nonint = 0, mn = 0xffffffffu, mx = 0u;
+    for (; i < count; i += stride) {
+        float v = x[i];
+        if (v != rintf(v)) nonint = 1;
+        unsigned o = ord32(v);
+        mn = min(mn, o);
+        mx = max(mx, o);
+    }
+    if (nonint) atomicOr(&out[0], 1u);
+    atomicMin(&out[1], mn);
+    atomicMax(&out[2], mx);
+}
+
+static float unord_host(unsigned o) {
+    unsigned u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+static int device_range(const float *d, uint64_t count, int *integer, float *vmin, float *vmax, hipStream_t st) {
+    unsigned init[3] = {0u, 0xffffffffu, 0u}, res[3];
+    unsigned *d_out = nullptr;
+    HIP_TRY(hipMalloc(&d_out, sizeof(init)));
+    HIP_TRY(hipMemcpyAsync(d_out, init, sizeof(init), hipMemcpyHostToDevice, st));
+    int blocks = (int)std::min<uint64_t>(4096, (count + 255) / 256 + 1);
+    hipLaunchKernelGGL(range_kernel, dim3(blocks), dim3(256), 0, st, d, count, d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(res, d_out, sizeof(res), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipFree(d_out));
+    *integer = res[0] == 0u;
+    *vmin = unord_host(res[1]);
+    *vmax = unord_host(res[2]);
+    return SONG_OK;
+}
+
+extern "C" int song_index_create(const float *data, uint64_t n, int dim, const uint64_t *graph, int shift,
+                                 int metric, song_index **out) {
+    if (!out) return fail(SONG_E_ARG, "out is null");
+    *out = nullptr;
+    if (!data || !graph) return fail(SONG_E_ARG, "data/graph is null");
+    if (n == 0 || n >= 0xFFFFFFFFull) return fail(SONG_E_ARG, "n=%llu out of range", (unsigned long long)n);
+    if (dim < 1 || dim > 16384) return fail(SONG_E_ARG, "dim=%d out of range", dim);
+    if (shift < 1 || shift > 6) return fail(SONG_E_ARG, "shift=%d not in [1,6]", shift);
+    if (metric < SONG_L2 || metric > SONG_COS) return fail(SONG_E_ARG, "metric=%d", metric);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SONG_E_NODEV, "no GPU visible");
+
+    const int slots = 1 << shift;
+    const int gstride = slots <= 32 ? 32 : 64;
+    std::vector<uint32_t> g32((size_t)n * gstride, 0u);
+    for (uint64_t v = 0; v < n; ++v) {
+        const uint64_t *r = graph + v * (uint64_t)slots;
+        uint64_t deg = r[0];
+        if (deg >= (uint64_t)slots)
+            return fail(SONG_E_ARG, "vertex %llu: degree %llu >= %d", (unsigned long long)v,
+                        (unsigned long long)deg, slots);
+        uint32_t *o = &g32[(size_t)v * gstride];
+        o[0] = (uint32_t)deg;
+        for (uint64_t j = 1; j <= deg; ++j) {
+            if (r[j] >= n)
+                return fail(SONG_E_ARG, "vertex %llu: neighbour id %llu >= n", (unsigned long long)v,
+                            (unsigned long long)r[j]);
+            o[j] = (uint32_t)r[j];
+        }
+    }
+
+    song_index *ix = new song_index();
+    ix->n = n;
+    ix->dim = dim;
+    ix->ld = (dim + 3) & ~3;
+    ix->metric = metric;
+    ix->shift = shift;
+    ix->gstride = gstride;
+    auto cleanup = [&](int code) {
+        song_index_destroy(ix);
+        return code;
+    };
+    if (hipGetDevice(&ix->device) != hipSuccess) return cleanup(fail(SONG_E_HIP, "hipGetDevice failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ix->device) == hipSuccess)
+        ix->max_lds = std::max<size_t>(65536, prop.sharedMemPerBlock);
+
+    size_t dbytes = (size_t)n * ix->ld * sizeof(float);
+    if (hipMalloc(&ix->d_data, dbytes) != hipSuccess) return cleanup(fail(SONG_E_HIP, "hipMalloc data %zu", dbytes));
+    if (hipMemset(ix->d_data, 0, dbytes) != hipSuccess) return cleanup(fail(SONG_E_HIP, "hipMemset data"));
+    if (hipMemcpy2D(ix->d_data, ix->ld * sizeof(float), data, dim * sizeof(float), dim * sizeof(float), n,
+                    hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(SONG_E_HIP, "upload data"));
+    size_t gbytes = g32.size() * sizeof(uint32_t);
+    if (hipMalloc(&ix->d_graph, gbytes) != hipSuccess) return cleanup(fail(SONG_E_HIP, "hipMalloc graph"));
+    if (hipMemcpy(ix->d_graph, g32.data(), gbytes, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(SONG_E_HIP, "upload graph"));
+    if (metric == SONG_COS) {
+        if (hipMalloc(&ix->d_norms, n * sizeof(float)) != hipSuccess) return cleanup(fail(SONG_E_HIP, "hipMalloc norms"));
+        hipLaunchKernelGGL(norms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, ix->d_data, n, dim,
+                           ix->ld, ix->d_norms);
+        if (hipGetLastError() != hipSuccess) return cleanup(fail(SONG_E_HIP, "norms kernel"));
+    }
+    int rc = device_range(ix->d_data, (uint64_t)n * ix->ld, &ix->integer_valued, &ix->vmin, &ix->vmax, 0);
+    if (rc != SONG_OK) return cleanup(rc);
+    if (hipDeviceSynchronize() != hipSuccess) return cleanup(fail(SONG_E_HIP, "index upload sync"));
+    *out = ix;
+    return SONG_OK;
+}
+
+extern "C" int song_index_destroy(song_index *ix) {
+    if (!ix) return SONG_OK;
+    if (ix->d_data) hipFree(ix->d_data);
+    if (ix->d_graph) hipFree(ix->d_graph);
+    if (ix->d_norms) hipFree(ix->d_norms);
+    delete ix;
+    return SONG_OK;
+}
+
+extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix->integer_valued : 0; }
+
+// ---------------------------------------------------------------- launch planning
+
+struct Plan {
+    bool exact;
+    int si, qi, qs, hlog, slack;
+    size_t lds;
+};
+
+static int sub_index(int ld) {
+    // lanes per row so that each lane reads about 8 floats of it
+    if (ld <= 32) return 0;   // 4 lanes
+    if (ld <= 256) return 1;  // 16 lanes
+    return 2;                 // 64 lanes
+}
+
+static size_t lds_bytes(bool exact, int qs, int hlog, int ld) {
+    size_t key = exact ? 16 : 8;
+    return ((size_t)4 << hlog) + 2 * (size_t)qs * 64 * key + 64 * key + 64 * 8 + (size_t)ld * 4;
+}
+
+static int make_plan(const song_index *ix, bool exact, int pq, int slack, int hlog, Plan *p) {
+    int need = pq + slack + 64;
+    int qi = -1;
+    for (int i = 0; i < kNumQs; ++i)
+        if (kQs[i] * 64 >= need) {
+            qi = i;
+            break;
+        }
+    if (qi < 0) return fail(SONG_E_ARG, "pq=%d + slack=%d exceeds the largest queue (%d)", pq, slack, kQs[kNumQs - 1] * 64 - 64);
+    p->exact = exact;
+    p->si = sub_index(ix->ld);
+    p->qi = qi;
+    p->qs = kQs[qi];
+    p->hlog = hlog;
+    p->slack = slack;
+    p->lds = lds_bytes(exact, p->qs, hlog, ix->ld);
+    if (p->lds > ix->max_lds)
+        return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
+    return SONG_OK;
+}
+
+static int default_hlog(int pq) {
+    int h = 9;
+    while ((1 << h) < 16 * pq && h < 14) ++h;
+    return h;
+}
+
+static SearchFn search_fn(int metric, const Plan &p) {
+    switch (metric) {
+    case SONG_L2: return search_fn_l2(p.exact, p.si, p.qi);
+    case SONG_IP: return search_fn_ip(p.exact, p.si, p.qi);
+    default: return search_fn_cos(p.exact, p.si, p.qi);
+    }
+}
+
+static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned grid, hipStream_t st) {
+    SearchFn fn = search_fn(ix->metric, p);
+    sp.hlog = p.hlog;
+    sp.slack = p.slack;
+    if (p.lds > 65536)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(64), p.lds, st, sp);
+    HIP_TRY(hipGetLastError());
+    return SONG_OK;
+}
+
+// FP32 accumulation is exact when every product and every partial sum is an integer
+// below 2^24: integer-valued data and queries with dim * span^2 < 2^24 (L2) or
+// dim * max|x| * max|q| < 2^24 (IP).
+static bool fp32_exact(const song_index *ix, int q_integer, float qmin, float qmax) {
+    if (!ix->integer_valued || !q_integer || ix->metric == SONG_COS) return false;
+    double lim = 16777216.0;
+    if (ix->metric == SONG_L2) {
+        double span = std::max((double)ix->vmax, (double)qmax) - std::min((double)ix->vmin, (double)qmin);
+        return (double)ix->dim * span * span < lim;
+    }
+    double ax = std::max(std::fabs((double)ix->vmin), std::fabs((double)ix->vmax));
+    double aq = std::max(std::fabs((double)qmin), std::fabs((double)qmax));
+    return (double)ix->dim * ax * aq < lim;
+}
+
+extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64_t nq, int pq, int k,
+                                  const int *d_pq, const int *d_k, uint32_t *d_ids, float *d_dist, int k_ld,
+                                  uint32_t *d_flags, uint32_t *d_stats, const song_search_opts *opts_in,
+                                  void *stream) {
+    if (!ix) return fail(SONG_E_ARG, "index is null");
+    if (nq == 0) return SONG_OK;
+    if (!d_queries || !d_ids) return fail(SONG_E_ARG, "queries/ids are null");
+    if (pq < 1 || k < 1 || k_ld < k) return fail(SONG_E_ARG, "pq=%d k=%d k_ld=%d", pq, k, k_ld);
+    if (nq > 0x7fffffffull) return fail(SONG_E_ARG, "nq too large");
+    song_search_opts opts;
+    song_search_opts_default(&opts);
+    if (opts_in) opts = *opts_in;
+    if (opts.slack < 0) opts.slack = 0;
+    if (opts.max_steps <= 0) opts.max_steps = 1000000;
+    hipStream_t st = (hipStream_t)stream;
+
+    bool exact = true;
+    if (opts.mode == SONG_MODE_FP32) {
+        exact = false;
+    } else if (opts.mode == SONG_MODE_AUTO && ix->integer_valued && ix->metric != SONG_COS) {
+        int qint = 0;
+        float qmin = 0.f, qmax = 0.f;
+        int rc = device_range(d_queries, nq * (uint64_t)ix->dim, &qint, &qmin, &qmax, st);
+        if (rc != SONG_OK) return rc;
+        exact = !fp32_exact(ix, qint, qmin, qmax);
+    }
+
+    int hlog = opts.hash_log2 > 0 ? opts.hash_log2 : default_hlog(pq);
+    hlog = std::max(9, std::min(hlog, 16));
+    Plan plan;
+    int rc = make_plan(ix, exact, pq, opts.slack, hlog, &plan);
+    if (rc != SONG_OK) return rc;
+
+    uint32_t *flags = d_flags;
+    uint32_t *tmp_flags = nullptr;
+    if (!flags && opts.retry) {
+        HIP_TRY(hipMalloc(&tmp_flags, nq * sizeof(uint32_t)));
+        flags = tmp_flags;
+    }
+    SearchParams sp{};
+    sp.data = ix->d_data;
+    sp.graph = ix->d_graph;
+    sp.norms = ix->d_norms;
+    sp.queries = d_queries;
+    sp.entry = nullptr;
+    sp.pq_arr = d_pq;
+    sp.k_arr = d_k;
+    sp.qidx = nullptr;
+    sp.out_ids = d_ids;
+    sp.out_dist = d_dist;
+    sp.out_flags = flags;
+    sp.out_stats = d_stats;
+    sp.n = ix->n;
+    sp.dim = ix->dim;
+    sp.ld = ix->ld;
+    sp.gstride = ix->gstride;
+    sp.nq = (int)nq;
+    sp.pq = pq;
+    sp.k = k;
+    sp.out_ld = k_ld;
+    sp.max_steps = opts.max_steps;
+    rc = launch(ix, plan, sp, (unsigned)nq, st);
+    if (rc == SONG_OK && opts.retry) {
+        // re-run queries whose LDS tables overflowed, with 4x larger tables, until none remain
+        std::vector<uint32_t> hf(nq);
+        uint32_t *d_idx = nullptr;
+        for (int round = 0; rc == SONG_OK; ++round) {
+            if (hipMemcpyAsync(hf.data(), flags, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                rc = fail(SONG_E_HIP, "flag readback");
+                break;
+            }
+            std::vector<uint32_t> redo;
+            bool vis = false, que = false;
+            for (uint64_t i = 0; i < nq; ++i)
+                if (hf[i] & (SONG_FLAG_VISITED_OVF | SONG_FLAG_QUEUE_OVF)) {
+                    redo.push_back((uint32_t)i);
+                    vis |= (hf[i] & SONG_FLAG_VISITED_OVF) != 0;
+                    que |= (hf[i] & SONG_FLAG_QUEUE_OVF) != 0;
+                }
+            if (redo.empty()) break;
+            if (round == 4) {
+                rc = fail(SONG_E_OVERFLOW, "%zu queries still overflow after 4 re-runs", redo.size());
+                break;
+            }
+            Plan p2;
+            int h2 = vis ? plan.hlog + 2 : plan.hlog;
+            int s2 = que ? plan.slack * 4 + 64 : plan.slack;
+            rc = make_plan(ix, exact, pq, s2, std::min(h2, 16), &p2);
+            if (rc != SONG_OK) break;
+            plan = p2;
+            if (!d_idx && hipMalloc(&d_idx, nq * sizeof(uint32_t)) != hipSuccess) {
+                rc = fail(SONG_E_HIP, "hipMalloc retry index");
+                break;
+            }
+            if (hipMemcpyAsync(d_idx, redo.data(), redo.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+                rc = fail(SONG_E_HIP, "retry index upload");
+                break;
+            }
+            SearchParams sp2 = sp;
+            sp2.qidx = d_idx;
+            rc = launch(ix, plan, sp2, (unsigned)redo.size(), st);
+        }
+        if (d_idx) hipFree(d_idx);
+    }
+    if (tmp_flags) hipFree(tmp_flags);
+    return rc;
+}
+
+extern "C" int song_search(song_index *ix, const float *queries, uint64_t nq, int pq, int k, uint64_t *out_ids,
+                           float *out_dist, uint32_t *out_flags, const song_search_opts *opts) {
+    if (!ix) return fail(SONG_E_ARG, "index is null");
+    if (nq == 0) return SONG_OK;
+    if (!queries || !out_ids) return fail(SONG_E_ARG, "queries/out_ids are null");
+    if (pq < 1 || k < 1) return fail(SONG_E_ARG, "pq=%d k=%d", pq, k);
+    k = std::min(k, pq);
+    float *d_q = nullptr, *d_d = nullptr;
+    uint32_t *d_i = nullptr, *d_f = nullptr;
+    int rc = SONG_OK;
+    size_t qb = nq * (size_t)ix->dim * sizeof(float), ib = nq * (size_t)k * sizeof(uint32_t);
+    if (hipMalloc(&d_q, qb) != hipSuccess || hipMalloc(&d_i, ib) != hipSuccess ||
+        hipMalloc(&d_f, nq * sizeof(uint32_t)) != hipSuccess || (out_dist && hipMalloc(&d_d, ib) != hipSuccess)) {
+        rc = fail(SONG_E_HIP, "hipMalloc search buffers");
+    }
+    if (rc == SONG_OK && hipMemcpy(d_q, queries, qb, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SONG_E_HIP, "upload queries");
+    if (rc == SONG_OK) rc = song_search_device(ix, d_q, nq, pq, k, nullptr, nullptr, d_i, d_d, k, d_f, nullptr, opts, nullptr);
+    if (rc == SONG_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(SONG_E_HIP, "search sync");
+    if (rc == SONG_OK) {
+        std::vector<uint32_t> ids(nq * (size_t)k);
+        if (hipMemcpy(ids.data(), d_i, ib, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(SONG_E_HIP, "download ids");
+        for (size_t i = 0; rc == SONG_OK && i < ids.size(); ++i)
+            out_ids[i] = ids[i] == 0xFFFFFFFFu ? UINT64_MAX : (uint64_t)ids[i];
+        if (rc == SONG_OK && out_dist && hipMemcpy(out_dist, d_d, ib, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(SONG_E_HIP, "download dists");
+        if (rc == SONG_OK && out_flags &&
+            hipMemcpy(out_flags, d_f, nq * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(SONG_E_HIP, "download flags");
+    }
+    if (d_q) hipFree(d_q);
+    if (d_i) hipFree(d_i);
+    if (d_d) hipFree(d_d);
+    if (d_f) hipFree(d_f);
+    return rc;
+}
+
+extern "C" int song_astar_multi_start_search_batch(const float *h_queries, uint64_t nq, int pq, uint64_t *results,
+                                                   const float *h_data, const uint64_t *h_graph,
+                                                   int vertex_offset_shift, uint64_t num, int dim, int metric) {
+    song_index *ix = nullptr;
+    int rc = song_index_create(h_data, num, dim, h_graph, vertex_offset_shift, metric, &ix);
+    if (rc != SONG_OK) return rc;
+    song_search_opts o;
+    song_search_opts_default(&o);
+    o.mode = SONG_MODE_EXACT;
+    rc = song_search(ix, h_queries, nq, pq, pq, results, nullptr, nullptr, &o);
+    song_index_destroy(ix);
+    return rc;
+}
