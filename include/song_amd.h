@@ -54,7 +54,6 @@ enum {
     SONG_FLAG_VISITED_OVF = 1u, /* visited table filled (query re-run with a larger table) */
     SONG_FLAG_QUEUE_OVF = 2u,   /* tie group at the pq-th distance exceeded the slack */
     SONG_FLAG_STEP_CAP = 4u,    /* hit the 1e6 expansion cap of graph.h:183 (same as the CPU) */
-    SONG_FLAG_RETRIED = 8u,     /* result comes from a re-run with larger tables */
 };
 
 typedef struct song_index song_index;
@@ -97,9 +96,12 @@ int song_search(song_index *index, const float *queries, uint64_t nq, int pq, in
  * Batched search with device buffers on `stream` (a hipStream_t, NULL = default).
  * d_queries nq x dim float32; d_pq / d_k: per-query budgets (nullable: use pq
  * and k for all); d_ids nq x k_ld uint32 (0xFFFFFFFF padding); d_dist nq x k_ld
- * float32 (nullable); d_flags nq uint32 (nullable); d_stats nq x 2 uint32
- * (expansions, distances computed; nullable).  pq is the maximum budget.
- * Runs asynchronously unless a retry is needed (then it synchronizes).
+ * float32 (nullable); d_flags nq uint32 (nullable); d_stats nq x 4 uint32
+ * (expansions, distances computed, adjacency words read = sum of 1 + degree
+ * over expanded vertices, 0; nullable).  pq is the maximum budget.
+ * With opts->retry set (the default) the call reads the flags back, re-runs
+ * overflowed queries and synchronizes the stream; with retry = 0 it only
+ * enqueues the search and the caller checks d_flags.
  */
 int song_search_device(song_index *index, const float *d_queries, uint64_t nq, int pq, int k,
                        const int *d_pq, const int *d_k, uint32_t *d_ids, float *d_dist, int k_ld,

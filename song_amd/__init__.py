@@ -141,3 +141,58 @@ class Index:
         _check(lib().song_search_device(self._h, dp(queries), nq, pq, k, dp(pq_arr), dp(k_arr), dp(ids), dp(dists),
                                         ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
                                         ctypes.c_void_p(stream)))
+
+
+# ---------------------------------------------------------------- host library (CPU side)
+
+HOST_LIB_PATH = os.path.join(_HERE, "lib", "libsong_host.so")
+_host = None
+
+
+class BuildStats(ctypes.Structure):
+    _fields_ = [("searches", ctypes.c_uint64), ("redone", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+
+
+HOST_EXPORTED_SYMBOLS = ("song_graph_build", "song_libsvm_parse", "song_file_write", "song_file_read",
+                         "song_host_last_error")
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise SongError(f"host library not built: {HOST_LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(HOST_LIB_PATH)
+        vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        L.song_host_last_error.restype = ctypes.c_char_p
+        L.song_graph_build.argtypes = [vp, u64, i32, i32, i32, vp, ctypes.POINTER(BuildStats)]
+        L.song_libsvm_parse.argtypes = [ctypes.c_char_p, i32, u64, i32, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.song_file_write.argtypes = [ctypes.c_char_p, vp, u64]
+        L.song_file_read.argtypes = [ctypes.c_char_p, vp, u64]
+        _host = L
+    return _host
+
+
+def _hcheck(rc):
+    if rc != 0:
+        raise SongError(f"song host error {rc}: {host_lib().song_host_last_error().decode(errors='replace')}")
+
+
+def build_graph(data, metric, threads=0, return_stats=False):
+    """Reference-identical insertion build (bfsg.graph image, uint64 [n, 32])."""
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    n, dim = d.shape
+    g = np.empty((n, 32), dtype=np.uint64)
+    st = BuildStats()
+    _hcheck(host_lib().song_graph_build(_ptr(d), n, dim, metric_code(metric), threads, _ptr(g), ctypes.byref(st)))
+    if return_stats:
+        return g, {"searches": st.searches, "redone": st.redone, "seconds": st.seconds}
+    return g
+
+
+def parse_libsvm(path, rows, dim, one_based=True):
+    out = np.empty((rows, dim), dtype=np.float32)
+    got = ctypes.c_uint64()
+    _hcheck(host_lib().song_libsvm_parse(path.encode(), 1 if one_based else 0, rows, dim, _ptr(out),
+                                         ctypes.byref(got)))
+    return out, int(got.value)

@@ -47,7 +47,7 @@ struct SearchParams {
     uint32_t *out_ids;      // nq x out_ld
     float *out_dist;        // nq x out_ld or null
     uint32_t *out_flags;    // nq or null
-    uint32_t *out_stats;    // nq x 2 or null
+    uint32_t *out_stats;    // nq x 4 or null: expansions, distances, sum(1+deg), 0
     uint64_t n;
     int dim, ld, gstride;
     int nq, pq, k, out_ld;
@@ -305,6 +305,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     uint32_t flags = 0u;
     int steps = 0;
     uint32_t nvis = 0u;
+    uint32_t adj_words = 0u;  // sum over expansions of (1 + degree): adjacency words read
     int h = 0, nq = 0, nbest = 0;
 
     // start vertex (graph.h:175-181)
@@ -362,6 +363,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         const uint32_t *row = P.graph + (size_t)now.id() * (size_t)P.gstride;
         const uint32_t r = (lane < P.gstride) ? row[lane] : 0u;
         const int deg = (int)rl32(r, 0);
+        adj_words += 1u + (uint32_t)deg;
         const bool fresh = (lane >= 1 && lane <= deg) ? vis_insert(vis, hmask, P.hlog, r) : false;
         const uint64_t fm = ballot(fresh);
         const int m = __popcll(fm);
@@ -464,8 +466,10 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     if (lane == 0) {
         if (P.out_flags) P.out_flags[qi] = flags;
         if (P.out_stats) {
-            P.out_stats[2 * (size_t)qi] = (uint32_t)steps;
-            P.out_stats[2 * (size_t)qi + 1] = nvis;
+            P.out_stats[4 * (size_t)qi] = (uint32_t)steps;
+            P.out_stats[4 * (size_t)qi + 1] = nvis;
+            P.out_stats[4 * (size_t)qi + 2] = adj_words;
+            P.out_stats[4 * (size_t)qi + 3] = 0u;
         }
     }
 }

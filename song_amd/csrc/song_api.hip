@@ -118,9 +118,6 @@ extern "C" int song_index_create(const float *data, uint64_t n, int dim, const u
     if (dim < 1 || dim > 16384) return fail(SONG_E_ARG, "dim=%d out of range", dim);
     if (shift < 1 || shift > 6) return fail(SONG_E_ARG, "shift=%d not in [1,6]", shift);
     if (metric < SONG_L2 || metric > SONG_COS) return fail(SONG_E_ARG, "metric=%d", metric);
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SONG_E_NODEV, "no GPU visible");
-
     const int slots = 1 << shift;
     const int gstride = slots <= 32 ? 32 : 64;
     std::vector<uint32_t> g32((size_t)n * gstride, 0u);
@@ -139,6 +136,9 @@ extern "C" int song_index_create(const float *data, uint64_t n, int dim, const u
             o[j] = (uint32_t)r[j];
         }
     }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SONG_E_NODEV, "no GPU visible");
 
     song_index *ix = new song_index();
     ix->n = n;

@@ -15,6 +15,7 @@ where fewer ids were printed).
 Tiny cases also store the whole graph as <case>_graph.npy.
 
 Usage: python tests/golden/make_golden.py [case ...]
+       python tests/golden/make_golden.py c2_1m <dir with song.cpu's 1M bfsg.data/bfsg.graph>
 """
 from __future__ import annotations
 
@@ -102,8 +103,38 @@ def run_case(name):
     print(name, meta["graph_sha256"][:16], flush=True)
 
 
+def run_c2_1m(prebuilt_dir, n_query=200, pqs=(100, 200)):
+    """The 1M x 128 mixture workload (seed 2).  Building it with song.cpu takes ~13 min,
+    so this reuses a directory where ``song.cpu build`` already wrote bfsg.data/bfsg.graph
+    from the same generator (checked through the data hash)."""
+    base, query = datasets.mixture_l2()
+    query = query[:n_query]
+    n, dim = base.shape
+    data_sha = sha256_file(os.path.join(prebuilt_dir, "bfsg.data"))
+    assert data_sha == __import__("hashlib").sha256(base.tobytes()).hexdigest(), "prebuilt data differs"
+    meta = {"case": "c2_1m", "generator": "mixture_l2", "kwargs": {}, "metric": "l2", "n_query": n_query,
+            "pqs": list(pqs), "graph_sha256": sha256_file(os.path.join(prebuilt_dir, "bfsg.graph")),
+            "data_sha256": data_sha,
+            "source": "reference song.cpu (main.cc, g++ -std=c++11 -O3 -march=native)"}
+    with tempfile.TemporaryDirectory() as d:
+        for f in ("bfsg.data", "bfsg.graph"):
+            os.symlink(os.path.join(os.path.abspath(prebuilt_dir), f), os.path.join(d, f))
+        datasets.write_libsvm(os.path.join(d, "query.svm"), query)
+        for pq in pqs:
+            out = subprocess.run([SONG_CPU, "test", "0", "query.svm", str(pq), str(n), str(dim), str(pq), "l2"],
+                                 cwd=d, check=True, capture_output=True, text=True).stdout
+            ids = np.array([[int(t) for t in line.split()] for line in out.split("\n")[:n_query]], dtype=np.uint32)
+            np.save(os.path.join(HERE, f"c2_1m_pq{pq}_ids.npy"), ids)
+    with open(os.path.join(HERE, "c2_1m.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("c2_1m", meta["graph_sha256"][:16], flush=True)
+
+
 if __name__ == "__main__":
     if not os.path.exists(SONG_CPU):
         sys.exit("build the reference binary first: make -C oracle ref")
+    if sys.argv[1:2] == ["c2_1m"]:
+        run_c2_1m(sys.argv[2])
+        sys.exit(0)
     for c in sys.argv[1:] or list(CASES):
         run_case(c)

@@ -33,7 +33,15 @@ GENERATORS = {
 
 
 def names():
-    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))
+    """Cases whose vectors, graph and ids fit a quick test (the 1M case has its own tests)."""
+    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json") and not f.startswith("c2_1m"))
+
+
+def load_c2_1m():
+    with open(os.path.join(GOLDEN, "c2_1m.json")) as f:
+        meta = json.load(f)
+    ids = {pq: np.load(os.path.join(GOLDEN, f"c2_1m_pq{pq}_ids.npy")).astype(np.int64) for pq in meta["pqs"]}
+    return meta, ids
 
 
 def load(name):

@@ -1,0 +1,73 @@
+"""GPU parity at the full 1M x 128 L2 workload (the mixture generator, seed 2).
+
+The graph is rebuilt here by the parallel builder and must hash equal to the
+reference song.cpu build; the GPU ids must equal the ids song.cpu printed for
+the first 200 queries at pq 100 and 200, in both arithmetic modes.  Over the
+whole 10k-query batch, the size-independent checks are: the exact and FP32
+modes agree, ids are distinct and ascending by distance, and recall rises with pq.
+"""
+import numpy as np
+import pytest
+
+import golden_cases
+import pyoracle
+import song_amd
+from song_amd import groundtruth, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2():
+    print("[c2_1m] loading workload + graph", flush=True)
+    meta, ref_ids = golden_cases.load_c2_1m()
+    base, query, graph, metric, st = workloads.load("c2_1m", verbose=True)
+    assert golden_cases.sha256(base) == meta["data_sha256"]
+    assert golden_cases.sha256(graph) == meta["graph_sha256"]
+    ix = song_amd.Index(base, graph, metric)
+    return meta, ref_ids, base, query, graph, ix
+
+
+@pytest.mark.parametrize("mode", [song_amd.MODE_EXACT, song_amd.MODE_AUTO])
+def test_c2_ids_equal_reference(c2, mode):
+    meta, ref_ids, base, query, graph, ix = c2
+    nq = meta["n_query"]
+    for pq in meta["pqs"]:
+        ids, ds, flags = ix.search(query[:nq], pq, mode=mode)
+        bad = np.nonzero((ids != ref_ids[pq]).any(1))[0]
+        assert len(bad) == 0, f"pq={pq}: {len(bad)} queries differ"
+
+
+def test_c2_full_batch_modes_agree_and_sorted(c2):
+    meta, _, base, query, graph, ix = c2
+    pq = 128
+    a, da, _ = ix.search(query, pq, 100, mode=song_amd.MODE_EXACT)
+    b, db, _ = ix.search(query, pq, 100, mode=song_amd.MODE_FP32)
+    assert np.array_equal(a, b)
+    assert np.array_equal(da, db)
+    assert (np.diff(da, axis=1) >= 0).all()
+    s = np.sort(a, axis=1)
+    assert (np.diff(s, axis=1) != 0).all()
+
+
+def test_c2_subset_matches_oracle_at_larger_pq(c2):
+    meta, _, base, query, graph, ix = c2
+    q = query[200:456]
+    for pq in (50, 400):
+        o, _ = pyoracle.search_batch(base, graph, "l2", q, pq, threads=8)
+        ids, _, _ = ix.search(q, pq)
+        assert np.array_equal(ids, o.astype(np.int64))
+
+
+def test_c2_recall_rises_with_pq(c2):
+    meta, _, base, query, graph, ix = c2
+    q = query[:2000]
+    gt_i, gt_d = groundtruth.knn_fp64(base, q, 10, "l2")
+    prev = 0.0
+    for pq in (16, 64, 256):
+        ids, _, _ = ix.search(q, pq, 10)
+        r = groundtruth.recall_at_k(ids, gt_d, 10, base, q, "l2")
+        print(f"[c2_1m] pq={pq} recall@10={r:.4f}", flush=True)
+        assert r >= prev
+        prev = r
+    assert prev > 0.8
