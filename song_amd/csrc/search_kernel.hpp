@@ -14,12 +14,18 @@
 //
 // Per-wave LDS layout (bytes, in order):
 //   visited  : H x u32 open-addressing set of (id + 1), linear probing, no deletion
-//   queue    : QS*64 keys, sorted ascending, live range [h, h + nq)
-//   best     : QS*64 keys, sorted ascending, first nbest live (pq-best expanded list)
-//   ckey     : 64 keys   candidate keys of the current expansion
+//   queue    : qcap = pq + slack + 64 keys, sorted ascending, live range [h, h + nq)
+//   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
 //   cid      : 64 u32    candidate ids of the current expansion
-//   cflag    : 64 u32    candidate needs sequential re-evaluation (exact mode)
 //   qv       : ld f32    the query, zero-padded to ld
+//
+// One wave per workgroup: LDS instructions of a wave execute in issue order, so lanes
+// exchange data through LDS with only a compiler fence between the write and the read
+// (lds_fence), and global loads stay in flight across those points.
+//
+// Per expansion the dependent global round trips are: adjacency row of the expanded
+// vertex (prefetched one step ahead for the predicted next vertex, the queue head after
+// the pop) and one gather of all new candidates' rows (all rows issued at once).
 //
 // Keys order by (distance, id) lexicographically:
 //   KeyF (fp32 mode):  one u64 = (orderable float bits << 32) | id
@@ -47,11 +53,11 @@ struct SearchParams {
     uint32_t *out_ids;      // nq x out_ld
     float *out_dist;        // nq x out_ld or null
     uint32_t *out_flags;    // nq or null
-    uint32_t *out_stats;    // nq x 4 or null: expansions, distances, sum(1+deg), 0
+    uint32_t *out_stats;    // nq x 4 or null: expansions, distances, sum(1+deg), prefetch hits
     uint64_t n;
     int dim, ld, gstride;
     int nq, pq, k, out_ld;
-    int hlog, slack, max_steps;
+    int hlog, slack, max_steps, qcap;
 };
 
 // ---------------------------------------------------------------- keys
@@ -118,7 +124,8 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lanes_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+// compiler-only fence for LDS traffic inside one wave (see the layout note above)
+__device__ __forceinline__ void lds_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
 // lowest set bit exponent of a nonzero float (value = odd * 2^e); INT_MAX for 0
 __device__ __forceinline__ int lsb_exp(float p) {
@@ -180,88 +187,108 @@ __device__ double seq_sum(const float *row, const float *qv, int dim) {
     return s;
 }
 
-// Distances of the m candidates cid[0..m) -> ckey[0..m).  SUB lanes share one row.
+// Distances of the m candidates cid[0..m): returns lane c's key (c < m) in `mine`.
+// SUB lanes share one row; each lane keeps its share of all rows of a pass in
+// registers (VMAX float4) so every row load of the pass is issued before the first use.
 template <int METRIC, bool EXACT, int SUB, class Key>
-__device__ void eval_candidates(const SearchParams &P, int m, const uint32_t *cid, Key *ckey, uint32_t *cflag,
-                                const float *qv, float lenv) {
+__device__ __forceinline__ Key eval_candidates(const SearchParams &P, int m, uint32_t cand, const uint32_t *cid,
+                                               const float *qv, float lenv) {
+    constexpr int VMAX = 16;
     const int lane = threadIdx.x;
     constexpr int G = 64 / SUB;
     const int g = lane / SUB, sl = lane % SUB;
     const int nv4 = P.ld >> 2;
     const float4 *q4 = reinterpret_cast<const float4 *>(qv);
-    bool need_seq = false;
+    Key mine = Key::inf();
+    bool redo = false;
     for (int base = 0; base < m; base += G) {
         const int c = base + g;
         const bool valid = c < m;
         const uint32_t id = valid ? cid[c] : 0u;
         const float4 *row = reinterpret_cast<const float4 *>(P.data + (size_t)id * (size_t)P.ld);
-        if constexpr (!EXACT) {
-            float acc = 0.0f;
-            if (valid) {
-                for (int j = sl; j < nv4; j += SUB) {
-                    float4 x = row[j], q = q4[j];
-                    if (METRIC == M_L2) {
-                        float a = x.x - q.x, b = x.y - q.y, cc = x.z - q.z, d = x.w - q.w;
-                        acc += a * a + b * b + cc * cc + d * d;
-                    } else {
-                        acc += x.x * q.x + x.y * q.y + x.z * q.z + x.w * q.w;
-                    }
-                }
+        float facc = 0.0f;
+        double dacc = 0.0, sab = 0.0;
+        int mn = 0x3fffffff;
+        for (int j0 = 0; j0 < nv4; j0 += VMAX * SUB) {
+            float4 x[VMAX];
+#pragma unroll
+            for (int v = 0; v < VMAX; ++v) {
+                const int j = j0 + sl + v * SUB;
+                if (valid && j < nv4) x[v] = row[j];
             }
 #pragma unroll
-            for (int o = SUB / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-            if (valid && sl == 0) {
-                float lena = (METRIC == M_COS) ? P.norms[id] : 0.0f;
-                ckey[c] = Key::make_f(finish_f32<METRIC>(acc, lena, lenv), id);
-            }
-        } else {
-            // float products (rounded), double accumulation; exact in any order when all
-            // products are multiples of 2^mn and their absolute sum is below 2^(mn+53).
-            double acc = 0.0, sab = 0.0;
-            int mn = 0x3fffffff;
-            if (valid) {
-                for (int j = sl; j < nv4; j += SUB) {
-                    float4 x = row[j], q = q4[j];
-                    float p0, p1, p2, p3;
-                    if (METRIC == M_L2) {
-                        float a = x.x - q.x, b = x.y - q.y, cc = x.z - q.z, d = x.w - q.w;
-                        p0 = a * a; p1 = b * b; p2 = cc * cc; p3 = d * d;
+            for (int v = 0; v < VMAX; ++v) {
+                const int j = j0 + sl + v * SUB;
+                if (valid && j < nv4) {
+                    const float4 q = q4[j];
+                    if constexpr (!EXACT) {
+                        if (METRIC == M_L2) {
+                            float a = x[v].x - q.x, b = x[v].y - q.y, cc = x[v].z - q.z, d = x[v].w - q.w;
+                            facc += a * a + b * b + cc * cc + d * d;
+                        } else {
+                            facc += x[v].x * q.x + x[v].y * q.y + x[v].z * q.z + x[v].w * q.w;
+                        }
                     } else {
-                        p0 = x.x * q.x; p1 = x.y * q.y; p2 = x.z * q.z; p3 = x.w * q.w;
+                        // float products (rounded), double accumulation; exact in any order when
+                        // all products are multiples of 2^mn and sum |p| < 2^(mn+53)
+                        float p0, p1, p2, p3;
+                        if (METRIC == M_L2) {
+                            float a = x[v].x - q.x, b = x[v].y - q.y, cc = x[v].z - q.z, d = x[v].w - q.w;
+                            p0 = a * a; p1 = b * b; p2 = cc * cc; p3 = d * d;
+                        } else {
+                            p0 = x[v].x * q.x; p1 = x[v].y * q.y; p2 = x[v].z * q.z; p3 = x[v].w * q.w;
+                        }
+                        dacc += (double)p0; dacc += (double)p1; dacc += (double)p2; dacc += (double)p3;
+                        sab += (double)fabsf(p0); sab += (double)fabsf(p1);
+                        sab += (double)fabsf(p2); sab += (double)fabsf(p3);
+                        mn = min(mn, min(min(lsb_exp(p0), lsb_exp(p1)), min(lsb_exp(p2), lsb_exp(p3))));
                     }
-                    acc += (double)p0; acc += (double)p1; acc += (double)p2; acc += (double)p3;
-                    sab += (double)fabsf(p0); sab += (double)fabsf(p1);
-                    sab += (double)fabsf(p2); sab += (double)fabsf(p3);
-                    mn = min(mn, min(min(lsb_exp(p0), lsb_exp(p1)), min(lsb_exp(p2), lsb_exp(p3))));
                 }
             }
+        }
+        Key k;
+        bool seq = false;
+        if constexpr (!EXACT) {
+#pragma unroll
+            for (int o = SUB / 2; o > 0; o >>= 1) facc += __shfl_xor(facc, o);
+            const float lena = (METRIC == M_COS && valid) ? P.norms[id] : 0.0f;
+            k = Key::make_f(finish_f32<METRIC>(facc, lena, lenv), id);
+        } else {
 #pragma unroll
             for (int o = SUB / 2; o > 0; o >>= 1) {
-                acc += __shfl_xor(acc, o);
+                dacc += __shfl_xor(dacc, o);
                 sab += __shfl_xor(sab, o);
                 mn = min(mn, __shfl_xor(mn, o));
             }
-            if (valid && sl == 0) {
-                bool exact = (sab == 0.0) || (mn < 0x3fffffff && sab < ldexp(1.0, mn + 53));
-                float lena = (METRIC == M_COS) ? P.norms[id] : 0.0f;
-                ckey[c] = Key::make_d(finish_exact<METRIC>(acc, lena, lenv), id);
-                cflag[c] = exact ? 0u : 1u;
-            }
-            need_seq = true;
+            seq = !((sab == 0.0) || (mn < 0x3fffffff && sab < ldexp(1.0, mn + 53)));
+            const float lena = (METRIC == M_COS && valid) ? P.norms[id] : 0.0f;
+            k = Key::make_d(finish_exact<METRIC>(dacc, lena, lenv), id);
+        }
+        // lane c (base <= c < base + G) takes the key of group c - base
+        const int src = ((lane - base) & (G - 1)) * SUB;
+        Key got;
+        if constexpr (!EXACT) {
+            got.v = ((uint64_t)(uint32_t)__shfl((int)(k.v >> 32), src) << 32) | (uint32_t)__shfl((int)k.v, src);
+        } else {
+            got.d = ((uint64_t)(uint32_t)__shfl((int)(k.d >> 32), src) << 32) | (uint32_t)__shfl((int)k.d, src);
+            got.i = (uint32_t)__shfl((int)k.i, src);
+            got.pad = 0u;
+        }
+        const bool seq_got = __shfl((int)seq, src) != 0;
+        if (lane >= base && lane < base + G && lane < m) {
+            mine = got;
+            redo = seq_got;
         }
     }
     if constexpr (EXACT) {
-      if (need_seq) {
-        wave_sync();
-        // sequential re-evaluation, one lane per flagged candidate
-        if (lane < m && cflag[lane]) {
-            const uint32_t id = cid[lane];
-            double s = seq_sum<METRIC>(P.data + (size_t)id * (size_t)P.ld, qv, P.dim);
-            float lena = (METRIC == M_COS) ? P.norms[id] : 0.0f;
-            ckey[lane] = Key::make_d(finish_exact<METRIC>(s, lena, lenv), id);
+        // sequential re-evaluation in the CPU's order, one lane per flagged candidate
+        if (redo) {
+            const double s = seq_sum<METRIC>(P.data + (size_t)cand * (size_t)P.ld, qv, P.dim);
+            const float lena = (METRIC == M_COS) ? P.norms[cand] : 0.0f;
+            mine = Key::make_d(finish_exact<METRIC>(s, lena, lenv), cand);
         }
-      }
     }
+    return mine;
 }
 
 // ---------------------------------------------------------------- the search kernel
@@ -269,7 +296,6 @@ __device__ void eval_candidates(const SearchParams &P, int m, const uint32_t *ci
 template <int METRIC, bool EXACT, int SUB, int QS>
 __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     using Key = typename KeySel<EXACT>::T;
-    constexpr int CAP = QS * 64;
     extern __shared__ __align__(16) unsigned char smem[];
 
     const int lane = threadIdx.x;
@@ -284,15 +310,13 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
 
     uint32_t *vis = reinterpret_cast<uint32_t *>(smem);
     Key *Q = reinterpret_cast<Key *>(smem + (size_t)H * 4u);
-    Key *B = Q + CAP;
-    Key *ckey = B + CAP;
-    uint32_t *cid = reinterpret_cast<uint32_t *>(ckey + 64);
-    uint32_t *cflag = cid + 64;
-    float *qv = reinterpret_cast<float *>(cflag + 64);
+    Key *B = Q + P.qcap;
+    uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));
+    float *qv = reinterpret_cast<float *>(cid + 64);
 
     for (uint32_t i = lane; i < H; i += 64) vis[i] = 0u;
     for (int i = lane; i < P.ld; i += 64) qv[i] = (i < P.dim) ? P.queries[(size_t)qi * P.dim + i] : 0.0f;
-    wave_sync();
+    lds_fence();
 
     float lenv = 0.0f;
     if (METRIC == M_COS) {
@@ -304,25 +328,27 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
 
     uint32_t flags = 0u;
     int steps = 0;
-    uint32_t nvis = 0u;
-    uint32_t adj_words = 0u;  // sum over expansions of (1 + degree): adjacency words read
+    uint32_t nvis = 0u, adj_words = 0u, pf_hits = 0u;
     int h = 0, nq = 0, nbest = 0;
+    uint64_t bmax = ~0ull;  // distance key of best[pq-1] once the list is full
 
     // start vertex (graph.h:175-181)
     const uint32_t start = P.entry ? P.entry[qi] : 0u;
-    if (P.n > 0) {
+    {
         if (lane == 0) {
             vis_insert(vis, hmask, P.hlog, start);
             cid[0] = start;
         }
         nvis = 1u;
-        wave_sync();
-        eval_candidates<METRIC, EXACT, SUB, Key>(P, 1, cid, ckey, cflag, qv, lenv);
-        wave_sync();
-        if (lane == 0) Q[0] = ckey[0];
+        lds_fence();
+        const Key k0 = eval_candidates<METRIC, EXACT, SUB, Key>(P, 1, start, cid, qv, lenv);
+        if (lane == 0) Q[0] = k0;
         nq = 1;
-        wave_sync();
+        lds_fence();
     }
+
+    uint32_t pf_id = 0xFFFFFFFFu;  // vertex whose adjacency row sits in pf_row
+    uint32_t pf_row = 0u;
 
     while (nq > 0) {
         if (steps >= P.max_steps) {
@@ -331,10 +357,26 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         }
         const Key now = Q[h];
         // stop rule, graph.h:188: |best| == pq and best.max.d < now.d (distance only, strict)
-        if (nbest == pq && B[pq - 1].dk() < now.dk()) break;
+        if (nbest == pq && bmax < now.dk()) break;
         ++steps;
         ++h;
         --nq;
+
+        // adjacency row of `now`: from the prefetch when it predicted this vertex
+        uint32_t r;
+        if (pf_id == now.id()) {
+            r = pf_row;
+            ++pf_hits;
+        } else {
+            r = (lane < P.gstride) ? P.graph[(size_t)now.id() * (size_t)P.gstride + lane] : 0u;
+        }
+        // prefetch the row of the new queue head, the likely next expansion
+        if (nq > 0) {
+            pf_id = Q[h].id();
+            pf_row = (lane < P.gstride) ? P.graph[(size_t)pf_id * (size_t)P.gstride + lane] : 0u;
+        } else {
+            pf_id = 0xFFFFFFFFu;
+        }
 
         // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195)
         {
@@ -343,11 +385,11 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
 #pragma unroll
             for (int s = 0; s < QS; ++s) {
                 const int i = lane + 64 * s;
-                t[s] = (i < nbest) ? B[i] : Key::inf();
-                below += __popcll(ballot(i < nbest && lt(t[s], now)));
+                const bool live = i < nbest;
+                t[s] = live ? B[i] : Key::inf();
+                below += __popcll(ballot(live && lt(t[s], now)));
             }
             if (below < pq) {
-                wave_sync();
 #pragma unroll
                 for (int s = 0; s < QS; ++s) {
                     const int i = lane + 64 * s;
@@ -355,13 +397,12 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
                 }
                 if (lane == 0) B[below] = now;
                 nbest = min(nbest + 1, pq);
-                wave_sync();
+                lds_fence();
+                if (nbest == pq) bmax = B[pq - 1].dk();
             }
         }
 
         // neighbours of `now` in row order (graph.h:196-205)
-        const uint32_t *row = P.graph + (size_t)now.id() * (size_t)P.gstride;
-        const uint32_t r = (lane < P.gstride) ? row[lane] : 0u;
         const int deg = (int)rl32(r, 0);
         adj_words += 1u + (uint32_t)deg;
         const bool fresh = (lane >= 1 && lane <= deg) ? vis_insert(vis, hmask, P.hlog, r) : false;
@@ -374,40 +415,18 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
             break;
         }
         if (m == 0) continue;
-        wave_sync();
-        eval_candidates<METRIC, EXACT, SUB, Key>(P, m, cid, ckey, cflag, qv, lenv);
-        wave_sync();
+        lds_fence();
+        const uint32_t cand = (lane < m) ? cid[lane] : 0u;
+        const Key mine = eval_candidates<METRIC, EXACT, SUB, Key>(P, m, cand, cid, qv, lenv);
 
         // ---- merge the candidates into the queue
-        uint64_t cut = ~0ull;  // drop anything with distance key > cut
-        if (nbest == pq) cut = B[pq - 1].dk();
+        uint64_t cut = (nbest == pq) ? bmax : ~0ull;  // drop distance keys above cut
         if (nq >= pq) cut = min(cut, Q[h + pq - 1].dk());
-        const Key mine = (lane < m) ? ckey[lane] : Key::inf();
         const bool surv = lane < m && mine.dk() <= cut;
         const uint64_t sm = ballot(surv);
         const int ms = __popcll(sm);
         if (ms == 0) continue;
 
-        // position of each survivor: rank among survivors + count of queue entries below it
-        int pos = 0;
-        {
-            uint64_t bits = sm;
-            while (bits) {
-                const int j = __builtin_ctzll(bits);
-                bits &= bits - 1;
-                const Key o = mine.rl(j);
-                pos += lt(o, mine) ? 1 : 0;
-            }
-            if (surv) {
-                int lo = 0, hi = nq;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (lt(Q[h + mid], mine)) lo = mid + 1;
-                    else hi = mid;
-                }
-                pos += lo;
-            }
-        }
         Key old[QS];
         int dst[QS];
 #pragma unroll
@@ -416,41 +435,68 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
             old[s] = (i < nq) ? Q[h + i] : Key::inf();
             dst[s] = i;
         }
+        int pos = 0;  // survivors: rank among survivors + queue entries below
         {
             uint64_t bits = sm;
             while (bits) {
                 const int j = __builtin_ctzll(bits);
                 bits &= bits - 1;
                 const Key o = mine.rl(j);
+                int below = 0;
 #pragma unroll
-                for (int s = 0; s < QS; ++s) dst[s] += lt(o, old[s]) ? 1 : 0;
+                for (int s = 0; s < QS; ++s) {
+                    const bool b = lt(old[s], o);  // inf entries are never below
+                    below += __popcll(ballot(b));
+                    dst[s] += b ? 0 : 1;
+                }
+                pos += (lt(o, mine) ? 1 : 0) + (lane == j ? below : 0);
             }
         }
-        wave_sync();
+        const int total = nq + ms;
+        // distance key of the entry that lands at position pq-1
+        uint64_t kth = ~0ull;
+        if (total > pq) {
+            uint64_t kv = ~0ull;
+            bool have = false;
+#pragma unroll
+            for (int s = 0; s < QS; ++s) {
+                const int i = lane + 64 * s;
+                if (i < nq && dst[s] == pq - 1) {
+                    kv = old[s].dk();
+                    have = true;
+                }
+            }
+            if (surv && pos == pq - 1) {
+                kv = mine.dk();
+                have = true;
+            }
+            const uint64_t hb = ballot(have);
+            kth = rl64(kv, __builtin_ctzll(hb));
+        }
+        const uint64_t cut2 = min((nbest == pq) ? bmax : ~0ull, kth);
 #pragma unroll
         for (int s = 0; s < QS; ++s) {
             const int i = lane + 64 * s;
             if (i < nq) Q[dst[s]] = old[s];
         }
         if (surv) Q[pos] = mine;
-        h = 0;
-        nq += ms;
-        wave_sync();
-
-        // truncate: keep entries with distance key <= cut2
-        uint64_t cut2 = (nbest == pq) ? B[pq - 1].dk() : ~0ull;
-        if (nq > pq) cut2 = min(cut2, Q[pq - 1].dk());
-        int keep = 0;
+        int keep = __popcll(ballot(surv && mine.dk() <= cut2));
 #pragma unroll
         for (int s = 0; s < QS; ++s) {
             const int i = lane + 64 * s;
-            const bool kp = i < nq && Q[i].dk() <= cut2;
-            keep += __popcll(ballot(kp));
+            keep += __popcll(ballot(i < nq && old[s].dk() <= cut2));
         }
+        h = 0;
         nq = keep;
+        lds_fence();
         if (nq > capq) {
             flags |= F_Q_OVF;
             break;
+        }
+        // the queue head may have changed: refresh the prefetch target only if it did
+        if (nq > 0 && Q[0].id() != pf_id) {
+            pf_id = Q[0].id();
+            pf_row = (lane < P.gstride) ? P.graph[(size_t)pf_id * (size_t)P.gstride + lane] : 0u;
         }
     }
 
@@ -469,15 +515,9 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
             P.out_stats[4 * (size_t)qi] = (uint32_t)steps;
             P.out_stats[4 * (size_t)qi + 1] = nvis;
             P.out_stats[4 * (size_t)qi + 2] = adj_words;
-            P.out_stats[4 * (size_t)qi + 3] = 0u;
+            P.out_stats[4 * (size_t)qi + 3] = pf_hits;
         }
     }
-}
-
-template <int METRIC, bool EXACT, int QS>
-constexpr size_t search_lds_bytes(int hlog, int ld) {
-    using Key = typename KeySel<EXACT>::T;
-    return ((size_t)4u << hlog) + 2u * QS * 64u * sizeof(Key) + 64u * sizeof(Key) + 64u * 8u + (size_t)ld * 4u;
 }
 
 }  // namespace song

@@ -194,20 +194,21 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 struct Plan {
     bool exact;
-    int si, qi, qs, hlog, slack;
+    int si, qi, qs, hlog, slack, qcap;
     size_t lds;
 };
 
 static int sub_index(int ld) {
-    // lanes per row so that each lane reads about 8 floats of it
-    if (ld <= 32) return 0;   // 4 lanes
-    if (ld <= 256) return 1;  // 16 lanes
-    return 2;                 // 64 lanes
+    // lanes per row: each lane holds at most 16 float4 of a row, so one pass covers
+    // 64/SUB candidates with all their loads in flight
+    if (ld <= 256) return 0;   // 4 lanes per row, 16 rows per pass
+    if (ld <= 1024) return 1;  // 16 lanes per row, 4 rows per pass
+    return 2;                  // 64 lanes per row
 }
 
-static size_t lds_bytes(bool exact, int qs, int hlog, int ld) {
+static size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int ld) {
     size_t key = exact ? 16 : 8;
-    return ((size_t)4 << hlog) + 2 * (size_t)qs * 64 * key + 64 * key + 64 * 8 + (size_t)ld * 4;
+    return ((size_t)4 << hlog) + (size_t)qcap * key + (size_t)((pq + 1) & ~1) * key + 64 * 4 + (size_t)ld * 4;
 }
 
 static int make_plan(const song_index *ix, bool exact, int pq, int slack, int hlog, Plan *p) {
@@ -225,17 +226,16 @@ static int make_plan(const song_index *ix, bool exact, int pq, int slack, int hl
     p->qs = kQs[qi];
     p->hlog = hlog;
     p->slack = slack;
-    p->lds = lds_bytes(exact, p->qs, hlog, ix->ld);
+    p->qcap = need;
+    p->lds = lds_bytes(exact, pq, need, hlog, ix->ld);
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
     return SONG_OK;
 }
 
-static int default_hlog(int pq) {
-    int h = 9;
-    while ((1 << h) < 16 * pq && h < 14) ++h;
-    return h;
-}
+// 4096 slots (16 KB) hold the ~1-3k distinct vertices a search discovers up to pq ~ 1000
+// at the 3/4 load limit; queries that discover more are re-run with larger tables.
+static int default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
 static SearchFn search_fn(int metric, const Plan &p) {
     switch (metric) {
@@ -249,6 +249,7 @@ static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned
     SearchFn fn = search_fn(ix->metric, p);
     sp.hlog = p.hlog;
     sp.slack = p.slack;
+    sp.qcap = p.qcap;
     if (p.lds > 65536)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
     hipLaunchKernelGGL(fn, dim3(grid), dim3(64), p.lds, st, sp);
