@@ -58,6 +58,9 @@ struct SearchParams {
     int dim, ld, gstride;
     int nq, pq, k, out_ld;
     int hlog, slack, max_steps, qcap;
+    uint32_t *gvis;         // per-launch-slot spill tables (2^ghlog u32 each) or null
+    int ghlog;
+    int qbase;              // first query (or qidx entry) of this launch
 };
 
 // ---------------------------------------------------------------- keys
@@ -115,6 +118,9 @@ __device__ __forceinline__ bool lt(const KeyD &a, const KeyD &b) {
     return a.d < b.d || (a.d == b.d && a.i < b.i);
 }
 
+__device__ __forceinline__ KeyF rl_key(const KeyF &k, int lane) { return k.rl(lane); }
+__device__ __forceinline__ KeyD rl_key(const KeyD &k, int lane) { return k.rl(lane); }
+
 template <bool EXACT> struct KeySel { using T = KeyF; };
 template <> struct KeySel<true> { using T = KeyD; };
 
@@ -145,6 +151,30 @@ __device__ __forceinline__ bool vis_insert(uint32_t *vis, uint32_t mask, int hlo
     uint32_t s = (id * 0x9E3779B1u) >> (32 - hlog);
     for (;;) {
         uint32_t old = atomicCAS(&vis[s], 0u, key);
+        if (old == 0u) return true;
+        if (old == key) return false;
+        s = (s + 1u) & mask;
+    }
+}
+
+// read-only probe of the LDS table
+__device__ __forceinline__ bool vis_test(const uint32_t *vis, uint32_t mask, int hlog, uint32_t id) {
+    const uint32_t key = id + 1u;
+    uint32_t s = (id * 0x9E3779B1u) >> (32 - hlog);
+    for (;;) {
+        const uint32_t v = vis[s];
+        if (v == key) return true;
+        if (v == 0u) return false;
+        s = (s + 1u) & mask;
+    }
+}
+
+// insert into this query's global spill table (same scheme, device-scope CAS)
+__device__ __forceinline__ bool gvis_insert(uint32_t *g, uint32_t mask, int hlog, uint32_t id) {
+    const uint32_t key = id + 1u;
+    uint32_t s = (id * 0x9E3779B1u) >> (32 - hlog);
+    for (;;) {
+        const uint32_t old = atomicCAS(&g[s], 0u, key);
         if (old == 0u) return true;
         if (old == key) return false;
         s = (s + 1u) & mask;
@@ -300,7 +330,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
 
     const int lane = threadIdx.x;
     const int slot_q = (int)blockIdx.x;
-    const int qi = P.qidx ? (int)P.qidx[slot_q] : slot_q;
+    const int qi = P.qidx ? (int)P.qidx[P.qbase + slot_q] : P.qbase + slot_q;
     const int pq = P.pq_arr ? P.pq_arr[qi] : P.pq;
     const int k = P.k_arr ? min(P.k_arr[qi], pq) : min(P.k, pq);
     const int capq = pq + P.slack;
@@ -329,6 +359,11 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     uint32_t flags = 0u;
     int steps = 0;
     uint32_t nvis = 0u, adj_words = 0u, pf_hits = 0u;
+    // visited spill: once the LDS table holds vis_limit - 64 ids, new ids go to a global table
+    bool spill = false;
+    uint32_t nglob = 0u;
+    uint32_t *gv = nullptr;
+    const uint32_t GH = 1u << P.ghlog, gmask = GH - 1u, glimit = (GH >> 2) * 3u;
     int h = 0, nq = 0, nbest = 0;
     uint64_t bmax = ~0ull;  // distance key of best[pq-1] once the list is full
 
@@ -378,8 +413,30 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
             pf_id = 0xFFFFFFFFu;
         }
 
-        // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195)
+        // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195).
+        // Fast path: `now` usually lands among the last 128 entries; look there first.
+        bool placed = false;
         {
+            const int t0 = max(0, nbest - 128);
+            const int i0 = t0 + lane, i1 = t0 + 64 + lane;
+            const bool l0 = i0 < nbest, l1 = i1 < nbest;
+            const Key e0 = l0 ? B[i0] : Key::inf();
+            const Key e1 = l1 ? B[i1] : Key::inf();
+            const bool front_below = (t0 == 0) || lt(Key(rl_key(e0, 0)), now);
+            if (front_below) {
+                const int below = t0 + __popcll(ballot(l0 && lt(e0, now))) + __popcll(ballot(l1 && lt(e1, now)));
+                if (below < pq) {
+                    if (l0 && i0 >= below && i0 + 1 < pq) B[i0 + 1] = e0;
+                    if (l1 && i1 >= below && i1 + 1 < pq) B[i1 + 1] = e1;
+                    if (lane == 0) B[below] = now;
+                    nbest = min(nbest + 1, pq);
+                    lds_fence();
+                    if (nbest == pq) bmax = B[pq - 1].dk();
+                }
+                placed = true;
+            }
+        }
+        if (!placed) {
             Key t[QS];
             int below = 0;
 #pragma unroll
@@ -405,14 +462,32 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         // neighbours of `now` in row order (graph.h:196-205)
         const int deg = (int)rl32(r, 0);
         adj_words += 1u + (uint32_t)deg;
-        const bool fresh = (lane >= 1 && lane <= deg) ? vis_insert(vis, hmask, P.hlog, r) : false;
+        if (!spill && nvis + 64u > vis_limit) {
+            if (!P.gvis) {
+                flags |= F_VIS_OVF;
+                break;
+            }
+            // switch to the global table for the rest of this query: clear it first
+            spill = true;
+            gv = P.gvis + (size_t)slot_q * GH;
+            for (uint32_t i = lane; i < GH; i += 64) gv[i] = 0u;
+            __threadfence_block();
+        }
+        bool fresh = false;
+        if (lane >= 1 && lane <= deg) {
+            if (!spill) fresh = vis_insert(vis, hmask, P.hlog, r);
+            else fresh = !vis_test(vis, hmask, P.hlog, r) && gvis_insert(gv, gmask, P.ghlog, r);
+        }
         const uint64_t fm = ballot(fresh);
         const int m = __popcll(fm);
         if (fresh) cid[lanes_below(fm)] = r;
         nvis += (uint32_t)m;
-        if (nvis > vis_limit) {
-            flags |= F_VIS_OVF;
-            break;
+        if (spill) {
+            nglob += (uint32_t)m;
+            if (nglob + 64u > glimit) {
+                flags |= F_VIS_OVF;
+                break;
+            }
         }
         if (m == 0) continue;
         lds_fence();

@@ -44,7 +44,23 @@ struct song_index {
     int integer_valued = 0;
     float vmin = 0.f, vmax = 0.f;
     size_t max_lds = 65536;
+    uint32_t *d_gvis = nullptr;  // visited spill tables, one per launch slot
+    size_t gvis_slots = 0;
 };
+
+// visited spill tables: 2^13 u32 (32 KB) per launch slot, launches of at most kChunk queries
+static constexpr int kGhlog = 13;
+static constexpr unsigned kChunk = 32768;
+
+static int ensure_gvis(song_index *ix, size_t slots) {
+    if (ix->gvis_slots >= slots) return SONG_OK;
+    if (ix->d_gvis) hipFree(ix->d_gvis);
+    ix->d_gvis = nullptr;
+    ix->gvis_slots = 0;
+    HIP_TRY(hipMalloc(&ix->d_gvis, slots * ((size_t)4 << kGhlog)));
+    ix->gvis_slots = slots;
+    return SONG_OK;
+}
 
 extern "C" const char *song_last_error(void) { return g_err.c_str(); }
 
@@ -184,6 +200,7 @@ extern "C" int song_index_destroy(song_index *ix) {
     if (ix->d_data) hipFree(ix->d_data);
     if (ix->d_graph) hipFree(ix->d_graph);
     if (ix->d_norms) hipFree(ix->d_norms);
+    if (ix->d_gvis) hipFree(ix->d_gvis);
     delete ix;
     return SONG_OK;
 }
@@ -237,6 +254,7 @@ static int make_plan(const song_index *ix, bool exact, int pq, int slack, int hl
 // at the 3/4 load limit; queries that discover more are re-run with larger tables.
 static int default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
+
 static SearchFn search_fn(int metric, const Plan &p) {
     switch (metric) {
     case SONG_L2: return search_fn_l2(p.exact, p.si, p.qi);
@@ -245,15 +263,20 @@ static SearchFn search_fn(int metric, const Plan &p) {
     }
 }
 
-static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned grid, hipStream_t st) {
+static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned count, hipStream_t st) {
     SearchFn fn = search_fn(ix->metric, p);
     sp.hlog = p.hlog;
     sp.slack = p.slack;
     sp.qcap = p.qcap;
+    sp.gvis = ix->d_gvis;
+    sp.ghlog = kGhlog;
     if (p.lds > 65536)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(64), p.lds, st, sp);
-    HIP_TRY(hipGetLastError());
+    for (unsigned q0 = 0; q0 < count; q0 += kChunk) {
+        sp.qbase = (int)q0;
+        hipLaunchKernelGGL(fn, dim3(std::min(kChunk, count - q0)), dim3(64), p.lds, st, sp);
+        HIP_TRY(hipGetLastError());
+    }
     return SONG_OK;
 }
 
@@ -299,6 +322,8 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
         exact = !fp32_exact(ix, qint, qmin, qmax);
     }
 
+    int rcg = ensure_gvis(ix, (size_t)std::min<uint64_t>(nq, kChunk));
+    if (rcg != SONG_OK) return rcg;
     int hlog = opts.hash_log2 > 0 ? opts.hash_log2 : default_hlog(pq);
     hlog = std::max(9, std::min(hlog, 16));
     Plan plan;

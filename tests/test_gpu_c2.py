@@ -38,6 +38,14 @@ def test_c2_ids_equal_reference(c2, mode):
         assert len(bad) == 0, f"pq={pq}: {len(bad)} queries differ"
 
 
+def test_c2_visited_spill_path_exact(c2):
+    """A 512-slot LDS table sends most queries through the global spill table."""
+    meta, ref_ids, base, query, graph, ix = c2
+    nq = meta["n_query"]
+    ids, _, flags = ix.search(query[:nq], 200, mode=song_amd.MODE_FP32, hash_log2=9)
+    assert np.array_equal(ids, ref_ids[200])
+
+
 def test_c2_full_batch_modes_agree_and_sorted(c2):
     meta, _, base, query, graph, ix = c2
     pq = 128

@@ -83,10 +83,11 @@ def test_auto_mode_on_integer_data_matches(case):
         assert np.array_equal(ids, ref_ids)
 
 
-def test_small_tables_force_retry_and_stay_exact():
+def test_small_tables_force_spill_and_retry_and_stay_exact():
     meta, base, query, ref_ids, graph = golden_cases.load("tie_l2_3k")
     ix = song_amd.Index(base, graph, meta["metric"])
-    # a 512-slot visited table and no tie slack overflow on purpose; the library re-runs
-    # those queries with larger tables and the ids must still equal the reference's
+    # a 512-slot visited table spills to the global table, and no tie slack overflows the
+    # queue on purpose (the library re-runs those queries with more slack); the ids must
+    # still equal the reference's
     ids, _, flags = ix.search(query, meta["pq"], mode=song_amd.MODE_EXACT, slack=0, hash_log2=9)
     assert np.array_equal(ids, ref_ids)
