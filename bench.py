@@ -54,7 +54,14 @@ def main():
     ap.add_argument("--pq", type=int, default=0, help="skip the sweep and time this pq")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-recall", action="store_true",
+                    help="with --pq: skip the ground truth and recall (profiling runs)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run one device copy of this many MiB first (a known-size kernel that "
+                         "calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.no_recall and not args.pq:
+        ap.error("--no-recall needs --pq")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries shard across GPUs "
                                    "with no communication, see DESIGN.md)"}))
@@ -73,9 +80,18 @@ def main():
     nq, dim = query.shape
     k = args.k
 
-    t1 = time.time()
-    gt_i, gt_d = groundtruth.knn_fp64(base, query, k, metric)
-    log(f"fp64 ground truth in {time.time() - t1:.1f}s")
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
+        del src, dst
+
+    if not args.no_recall:
+        t1 = time.time()
+        gt_i, gt_d = groundtruth.knn_fp64(base, query, k, metric)
+        log(f"fp64 ground truth in {time.time() - t1:.1f}s")
 
     q = torch.from_numpy(query).to(dev)
     stream = torch.cuda.current_stream()
@@ -93,7 +109,8 @@ def main():
     for pq in pqs:
         run(pq, ids, stats)
         torch.cuda.synchronize()
-        r = groundtruth.recall_at_k(ids.cpu().numpy().astype(np.int64), gt_d, k, base, query, metric)
+        r = (float("nan") if args.no_recall else
+             groundtruth.recall_at_k(ids.cpu().numpy().astype(np.int64), gt_d, k, base, query, metric))
         st = stats.cpu().numpy()
         times = []
         for _ in range(3):

@@ -2,7 +2,7 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_cos(bool exact, int si, int qi) {
-    return exact ? pick_sub<M_COS, true>(si, qi) : pick_sub<M_COS, false>(si, qi);
+SearchFn search_fn_cos(bool exact, int shape) {
+    return exact ? pick_shape<M_COS, true>(shape) : pick_shape<M_COS, false>(shape);
 }
 }  // namespace song

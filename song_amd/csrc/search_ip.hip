@@ -2,7 +2,7 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_ip(bool exact, int si, int qi) {
-    return exact ? pick_sub<M_IP, true>(si, qi) : pick_sub<M_IP, false>(si, qi);
+SearchFn search_fn_ip(bool exact, int shape) {
+    return exact ? pick_shape<M_IP, true>(shape) : pick_shape<M_IP, false>(shape);
 }
 }  // namespace song

@@ -14,7 +14,7 @@
 //
 // Per-wave LDS layout (bytes, in order):
 //   visited  : H x u32 open-addressing set of (id + 1), linear probing, no deletion
-//   queue    : qcap = pq + slack + 64 keys, sorted ascending, live range [h, h + nq)
+//   queue    : qcap = pq + slack + 128 keys, sorted ascending, live range [h, h + nq), h < 64
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
 //   cid      : 64 u32    candidate ids of the current expansion
 //   qv       : ld f32    the query, zero-padded to ld
@@ -217,13 +217,12 @@ __device__ double seq_sum(const float *row, const float *qv, int dim) {
     return s;
 }
 
-// Distances of the m candidates cid[0..m): returns lane c's key (c < m) in `mine`.
+// Distances of the m candidates cid[0..m): returns lane c's key (c < m).
 // SUB lanes share one row; each lane keeps its share of all rows of a pass in
-// registers (VMAX float4) so every row load of the pass is issued before the first use.
-template <int METRIC, bool EXACT, int SUB, class Key>
+// registers (up to VMAX float4) so every row load of the pass is issued before the first use.
+template <int METRIC, bool EXACT, int SUB, int VMAX, class Key>
 __device__ __forceinline__ Key eval_candidates(const SearchParams &P, int m, uint32_t cand, const uint32_t *cid,
                                                const float *qv, float lenv) {
-    constexpr int VMAX = 16;
     const int lane = threadIdx.x;
     constexpr int G = 64 / SUB;
     const int g = lane / SUB, sl = lane % SUB;
@@ -323,9 +322,42 @@ __device__ __forceinline__ Key eval_candidates(const SearchParams &P, int m, uin
 
 // ---------------------------------------------------------------- the search kernel
 
-template <int METRIC, bool EXACT, int SUB, int QS>
+// Count of the leading entries of the sorted run arr[0..n) that satisfy `below`, a
+// predicate that holds on a prefix.  64-ary search: each round samples the last entry of
+// 64 equal chunks (one LDS read per lane) and keeps the chunk holding the boundary.
+template <class Key, class Pred>
+__device__ __forceinline__ int prefix_count(const Key *arr, int n, Pred below) {
+    const int lane = threadIdx.x;
+    int lo = 0, len = n;
+    while (len > 64) {
+        const int stride = (len + 63) >> 6;
+        const int end = lo + len;
+        const int idx = min(lo + (lane + 1) * stride, end) - 1;
+        const bool v = lo + lane * stride < end;
+        const Key e = arr[v ? idx : lo];
+        const int c = __popcll(ballot(v && below(e)));
+        lo = min(lo + c * stride, end);  // the last chunk may be partial
+        len = min(stride, end - lo);
+    }
+    const bool v = lane < len;
+    const Key e = arr[v ? lo + lane : 0];
+    return lo + __popcll(ballot(v && below(e)));
+}
+
+// shapes: (lanes per candidate row, float4 registers per lane) by padded row width
+template <int SHAPE> struct Shape;
+template <> struct Shape<0> { static constexpr int SUB = 4, NV = 2; };    // ld <= 32
+template <> struct Shape<1> { static constexpr int SUB = 4, NV = 8; };    // ld <= 128
+template <> struct Shape<2> { static constexpr int SUB = 4, NV = 16; };   // ld <= 256
+template <> struct Shape<3> { static constexpr int SUB = 16, NV = 16; };  // ld <= 1024
+template <> struct Shape<4> { static constexpr int SUB = 64, NV = 16; };  // wider rows, chunked
+
+// ---------------------------------------------------------------- the search kernel
+
+template <int METRIC, bool EXACT, int SHAPE>
 __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     using Key = typename KeySel<EXACT>::T;
+    constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
     extern __shared__ __align__(16) unsigned char smem[];
 
     const int lane = threadIdx.x;
@@ -359,7 +391,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     uint32_t flags = 0u;
     int steps = 0;
     uint32_t nvis = 0u, adj_words = 0u, pf_hits = 0u;
-    // visited spill: once the LDS table holds vis_limit - 64 ids, new ids go to a global table
+    // visited spill: once the LDS table nears its load limit, new ids go to a global table
     bool spill = false;
     uint32_t nglob = 0u;
     uint32_t *gv = nullptr;
@@ -376,7 +408,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         }
         nvis = 1u;
         lds_fence();
-        const Key k0 = eval_candidates<METRIC, EXACT, SUB, Key>(P, 1, start, cid, qv, lenv);
+        const Key k0 = eval_candidates<METRIC, EXACT, SUB, NV, Key>(P, 1, start, cid, qv, lenv);
         if (lane == 0) Q[0] = k0;
         nq = 1;
         lds_fence();
@@ -413,46 +445,19 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
             pf_id = 0xFFFFFFFFu;
         }
 
-        // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195).
-        // Fast path: `now` usually lands among the last 128 entries; look there first.
-        bool placed = false;
+        // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195):
+        // find its position, shift the entries behind it one place (last block first)
         {
-            const int t0 = max(0, nbest - 128);
-            const int i0 = t0 + lane, i1 = t0 + 64 + lane;
-            const bool l0 = i0 < nbest, l1 = i1 < nbest;
-            const Key e0 = l0 ? B[i0] : Key::inf();
-            const Key e1 = l1 ? B[i1] : Key::inf();
-            const bool front_below = (t0 == 0) || lt(Key(rl_key(e0, 0)), now);
-            if (front_below) {
-                const int below = t0 + __popcll(ballot(l0 && lt(e0, now))) + __popcll(ballot(l1 && lt(e1, now)));
-                if (below < pq) {
-                    if (l0 && i0 >= below && i0 + 1 < pq) B[i0 + 1] = e0;
-                    if (l1 && i1 >= below && i1 + 1 < pq) B[i1 + 1] = e1;
-                    if (lane == 0) B[below] = now;
-                    nbest = min(nbest + 1, pq);
-                    lds_fence();
-                    if (nbest == pq) bmax = B[pq - 1].dk();
+            const int p = prefix_count(B, nbest, [&](const Key &e) { return lt(e, now); });
+            if (p < pq) {
+                const int last = min(nbest, pq - 1);  // entries [p, last) move to [p+1, last+1)
+                for (int blk = (last - 1) >> 6; blk >= (p >> 6) && last > p; --blk) {
+                    const int i = blk * 64 + lane;
+                    const bool live = i >= p && i < last;
+                    const Key e = B[live ? i : 0];
+                    if (live) B[i + 1] = e;
                 }
-                placed = true;
-            }
-        }
-        if (!placed) {
-            Key t[QS];
-            int below = 0;
-#pragma unroll
-            for (int s = 0; s < QS; ++s) {
-                const int i = lane + 64 * s;
-                const bool live = i < nbest;
-                t[s] = live ? B[i] : Key::inf();
-                below += __popcll(ballot(live && lt(t[s], now)));
-            }
-            if (below < pq) {
-#pragma unroll
-                for (int s = 0; s < QS; ++s) {
-                    const int i = lane + 64 * s;
-                    if (i >= below && i < nbest && i + 1 < pq) B[i + 1] = t[s];
-                }
-                if (lane == 0) B[below] = now;
+                if (lane == 0) B[p] = now;
                 nbest = min(nbest + 1, pq);
                 lds_fence();
                 if (nbest == pq) bmax = B[pq - 1].dk();
@@ -492,7 +497,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         if (m == 0) continue;
         lds_fence();
         const uint32_t cand = (lane < m) ? cid[lane] : 0u;
-        const Key mine = eval_candidates<METRIC, EXACT, SUB, Key>(P, m, cand, cid, qv, lenv);
+        const Key mine = eval_candidates<METRIC, EXACT, SUB, NV, Key>(P, m, cand, cid, qv, lenv);
 
         // ---- merge the candidates into the queue
         uint64_t cut = (nbest == pq) ? bmax : ~0ull;  // drop distance keys above cut
@@ -502,75 +507,56 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         const int ms = __popcll(sm);
         if (ms == 0) continue;
 
-        Key old[QS];
-        int dst[QS];
-#pragma unroll
-        for (int s = 0; s < QS; ++s) {
-            const int i = lane + 64 * s;
-            old[s] = (i < nq) ? Q[h + i] : Key::inf();
-            dst[s] = i;
+        if (h >= 64) {
+            // move the live range back to the buffer start (left moves, first block first)
+            for (int blk = 0; blk * 64 < nq; ++blk) {
+                const int i = blk * 64 + lane;
+                const Key e = Q[h + min(i, nq - 1)];
+                if (i < nq) Q[i] = e;
+            }
+            h = 0;
+            lds_fence();
         }
-        int pos = 0;  // survivors: rank among survivors + queue entries below
-        {
-            uint64_t bits = sm;
-            while (bits) {
+        Key smin = Key::inf();
+        int rank = 0;  // survivors below this lane's survivor
+        for (uint64_t bits = sm; bits; bits &= bits - 1) {
+            const Key o = mine.rl(__builtin_ctzll(bits));
+            if (lt(o, smin)) smin = o;
+            rank += lt(o, mine) ? 1 : 0;
+        }
+        // entries [0, p0) stay; entries [p0, nq) move right by the survivors below them
+        const int p0 = prefix_count(Q + h, nq, [&](const Key &e) { return lt(e, smin); });
+        int acc = 0;  // this lane's survivor: queue entries in [p0, nq) below it
+        for (int blk = (nq - 1) >> 6; nq > p0 && blk >= (p0 >> 6); --blk) {
+            const int i = blk * 64 + lane;
+            const bool live = i >= p0 && i < nq;
+            const Key e = live ? Q[h + i] : Key::inf();
+            int cnt = 0;
+            for (uint64_t bits = sm; bits; bits &= bits - 1) {
                 const int j = __builtin_ctzll(bits);
-                bits &= bits - 1;
                 const Key o = mine.rl(j);
-                int below = 0;
-#pragma unroll
-                for (int s = 0; s < QS; ++s) {
-                    const bool b = lt(old[s], o);  // inf entries are never below
-                    below += __popcll(ballot(b));
-                    dst[s] += b ? 0 : 1;
-                }
-                pos += (lt(o, mine) ? 1 : 0) + (lane == j ? below : 0);
+                const bool sb = lt(o, e);
+                cnt += sb ? 1 : 0;
+                const int below = __popcll(ballot(live && !sb));
+                if (lane == j) acc += below;
             }
+            if (live) Q[h + i + cnt] = e;
         }
-        const int total = nq + ms;
-        // distance key of the entry that lands at position pq-1
-        uint64_t kth = ~0ull;
-        if (total > pq) {
-            uint64_t kv = ~0ull;
-            bool have = false;
-#pragma unroll
-            for (int s = 0; s < QS; ++s) {
-                const int i = lane + 64 * s;
-                if (i < nq && dst[s] == pq - 1) {
-                    kv = old[s].dk();
-                    have = true;
-                }
-            }
-            if (surv && pos == pq - 1) {
-                kv = mine.dk();
-                have = true;
-            }
-            const uint64_t hb = ballot(have);
-            kth = rl64(kv, __builtin_ctzll(hb));
-        }
-        const uint64_t cut2 = min((nbest == pq) ? bmax : ~0ull, kth);
-#pragma unroll
-        for (int s = 0; s < QS; ++s) {
-            const int i = lane + 64 * s;
-            if (i < nq) Q[dst[s]] = old[s];
-        }
-        if (surv) Q[pos] = mine;
-        int keep = __popcll(ballot(surv && mine.dk() <= cut2));
-#pragma unroll
-        for (int s = 0; s < QS; ++s) {
-            const int i = lane + 64 * s;
-            keep += __popcll(ballot(i < nq && old[s].dk() <= cut2));
-        }
-        h = 0;
-        nq = keep;
+        if (surv) Q[h + p0 + acc + rank] = mine;
+        nq += ms;
         lds_fence();
+
+        // truncate: keep entries with distance key <= cut2
+        uint64_t cut2 = (nbest == pq) ? bmax : ~0ull;
+        if (nq > pq) cut2 = min(cut2, Q[h + pq - 1].dk());
+        if (cut2 != ~0ull) nq = prefix_count(Q + h, nq, [&](const Key &e) { return e.dk() <= cut2; });
         if (nq > capq) {
             flags |= F_Q_OVF;
             break;
         }
         // the queue head may have changed: refresh the prefetch target only if it did
-        if (nq > 0 && Q[0].id() != pf_id) {
-            pf_id = Q[0].id();
+        if (nq > 0 && Q[h].id() != pf_id) {
+            pf_id = Q[h].id();
             pf_row = (lane < P.gstride) ? P.graph[(size_t)pf_id * (size_t)P.gstride + lane] : 0u;
         }
     }

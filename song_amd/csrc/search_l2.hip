@@ -2,7 +2,7 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_l2(bool exact, int si, int qi) {
-    return exact ? pick_sub<M_L2, true>(si, qi) : pick_sub<M_L2, false>(si, qi);
+SearchFn search_fn_l2(bool exact, int shape) {
+    return exact ? pick_shape<M_L2, true>(shape) : pick_shape<M_L2, false>(shape);
 }
 }  // namespace song

@@ -211,17 +211,9 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 struct Plan {
     bool exact;
-    int si, qi, qs, hlog, slack, qcap;
+    int shape, hlog, slack, qcap;
     size_t lds;
 };
-
-static int sub_index(int ld) {
-    // lanes per row: each lane holds at most 16 float4 of a row, so one pass covers
-    // 64/SUB candidates with all their loads in flight
-    if (ld <= 256) return 0;   // 4 lanes per row, 16 rows per pass
-    if (ld <= 1024) return 1;  // 16 lanes per row, 4 rows per pass
-    return 2;                  // 64 lanes per row
-}
 
 static size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int ld) {
     size_t key = exact ? 16 : 8;
@@ -229,22 +221,14 @@ static size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int ld) {
 }
 
 static int make_plan(const song_index *ix, bool exact, int pq, int slack, int hlog, Plan *p) {
-    int need = pq + slack + 64;
-    int qi = -1;
-    for (int i = 0; i < kNumQs; ++i)
-        if (kQs[i] * 64 >= need) {
-            qi = i;
-            break;
-        }
-    if (qi < 0) return fail(SONG_E_ARG, "pq=%d + slack=%d exceeds the largest queue (%d)", pq, slack, kQs[kNumQs - 1] * 64 - 64);
+    if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
     p->exact = exact;
-    p->si = sub_index(ix->ld);
-    p->qi = qi;
-    p->qs = kQs[qi];
+    p->shape = shape_for_ld(ix->ld);
     p->hlog = hlog;
     p->slack = slack;
-    p->qcap = need;
-    p->lds = lds_bytes(exact, pq, need, hlog, ix->ld);
+    // queue buffer: pq + slack live entries, 64 of merge room, 64 of head room
+    p->qcap = pq + slack + 128;
+    p->lds = lds_bytes(exact, pq, p->qcap, hlog, ix->ld);
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
     return SONG_OK;
@@ -257,9 +241,9 @@ static int default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
 static SearchFn search_fn(int metric, const Plan &p) {
     switch (metric) {
-    case SONG_L2: return search_fn_l2(p.exact, p.si, p.qi);
-    case SONG_IP: return search_fn_ip(p.exact, p.si, p.qi);
-    default: return search_fn_cos(p.exact, p.si, p.qi);
+    case SONG_L2: return search_fn_l2(p.exact, p.shape);
+    case SONG_IP: return search_fn_ip(p.exact, p.shape);
+    default: return search_fn_cos(p.exact, p.shape);
     }
 }
 
