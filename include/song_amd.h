@@ -58,12 +58,23 @@ enum {
 
 typedef struct song_index song_index;
 
+/*
+ * Row storage read by the search (precision form).  SONG_STORE_F32 reads the float32
+ * rows.  BF16 / FP8 read a low-precision copy built by song_index_add_storage: code *
+ * scale[row], scale a power of two for BF16 (lossless for integers up to 256) and
+ * max|x| / 448 for FP8-E4M3; distances accumulate in FP32 and the final pq list is
+ * re-ranked with exact FP32 distances (opts.rerank).
+ */
+enum { SONG_STORE_F32 = 0, SONG_STORE_BF16 = 1, SONG_STORE_FP8 = 2 };
+
 typedef struct {
-    int mode;      /* SONG_MODE_* (default AUTO) */
+    int mode;      /* SONG_MODE_* (default AUTO); BF16/FP8 storage always accumulates in FP32 */
     int slack;     /* queue entries kept beyond pq for distance ties (default 32) */
     int hash_log2; /* visited-table slots = 2^hash_log2; 0 = sized from pq */
     int max_steps; /* expansion cap; 0 = 1,000,000 (graph.h:183) */
     int retry;     /* re-run overflowed queries with larger tables (default 1) */
+    int storage;   /* SONG_STORE_* (default F32) */
+    int rerank;    /* low-precision storage: exact FP32 re-rank of the pq list (default 1) */
 } song_search_opts;
 
 /* Fill opts with the defaults above. */
@@ -81,6 +92,15 @@ int song_index_destroy(song_index *index);
 
 /* 1 if every base coordinate is an integer (FP32 mode can be exact), else 0. */
 int song_index_integer_valued(const song_index *index);
+
+/* Build the BF16 or FP8-E4M3 copy of the rows (with per-row scales) on the GPU. */
+int song_index_add_storage(song_index *index, int storage);
+
+/*
+ * Copy back a low-precision storage for inspection: codes n x row_bytes (row_bytes =
+ * dim rounded up to 8 BF16 / 16 FP8 elements, times the element size) and scales n.
+ */
+int song_index_get_storage(const song_index *index, int storage, void *codes, float *scales, uint64_t *row_bytes);
 
 /*
  * Batched search with host buffers.  queries: nq x dim float32 (dense,

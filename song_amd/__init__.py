@@ -18,6 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libsong_hip.so")
 
 MODE_AUTO, MODE_EXACT, MODE_FP32 = 0, 1, 2
+STORE_F32, STORE_BF16, STORE_FP8 = 0, 1, 2
 FLAG_VISITED_OVF, FLAG_QUEUE_OVF, FLAG_STEP_CAP = 1, 2, 4
 
 _lib = None
@@ -29,7 +30,8 @@ class SongError(RuntimeError):
 
 class SearchOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slack", ctypes.c_int), ("hash_log2", ctypes.c_int),
-                ("max_steps", ctypes.c_int), ("retry", ctypes.c_int)]
+                ("max_steps", ctypes.c_int), ("retry", ctypes.c_int), ("storage", ctypes.c_int),
+                ("rerank", ctypes.c_int)]
 
 
 def lib():
@@ -56,6 +58,8 @@ def lib():
         L.song_search_device.argtypes = [vp, vp, u64, i32, i32, vp, vp, vp, vp, i32, vp, vp,
                                          ctypes.POINTER(SearchOpts), vp]
         L.song_astar_multi_start_search_batch.argtypes = [vp, u64, i32, vp, vp, vp, i32, u64, i32, i32]
+        L.song_index_add_storage.argtypes = [vp, i32]
+        L.song_index_get_storage.argtypes = [vp, i32, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -63,6 +67,7 @@ def lib():
 EXPORTED_SYMBOLS = (
     "song_search_opts_default", "song_index_create", "song_index_destroy", "song_index_integer_valued",
     "song_search", "song_search_device", "song_astar_multi_start_search_batch", "song_last_error",
+    "song_index_add_storage", "song_index_get_storage",
 )
 
 
@@ -79,10 +84,11 @@ def metric_code(metric):
     return METRIC_NAMES[metric] if isinstance(metric, str) else int(metric)
 
 
-def make_opts(mode=MODE_AUTO, slack=32, hash_log2=0, max_steps=0, retry=1):
+def make_opts(mode=MODE_AUTO, slack=32, hash_log2=0, max_steps=0, retry=1, storage=STORE_F32, rerank=1):
     o = SearchOpts()
     lib().song_search_opts_default(ctypes.byref(o))
     o.mode, o.slack, o.hash_log2, o.max_steps, o.retry = mode, slack, hash_log2, max_steps, retry
+    o.storage, o.rerank = storage, rerank
     return o
 
 
@@ -114,7 +120,21 @@ class Index:
     def integer_valued(self):
         return bool(lib().song_index_integer_valued(self._h))
 
-    def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=32, hash_log2=0, retry=1):
+    def add_storage(self, storage):
+        """Build the BF16 / FP8 copy of the rows on the GPU."""
+        _check(lib().song_index_add_storage(self._h, storage))
+
+    def get_storage(self, storage):
+        """(codes uint8 [n, row_bytes], scales float32 [n]) of a built low-precision copy."""
+        rb = ctypes.c_uint64()
+        _check(lib().song_index_get_storage(self._h, storage, None, None, ctypes.byref(rb)))
+        codes = np.empty((self.n, rb.value), dtype=np.uint8)
+        scales = np.empty(self.n, dtype=np.float32)
+        _check(lib().song_index_get_storage(self._h, storage, _ptr(codes), _ptr(scales), ctypes.byref(rb)))
+        return codes, scales
+
+    def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, storage=STORE_F32,
+               rerank=1):
         """Host buffers in and out: ids int64 [nq, k] (-1 padding), dists float32, flags uint32."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
@@ -122,14 +142,15 @@ class Index:
         ids = np.empty((nq, k), dtype=np.uint64)
         ds = np.empty((nq, k), dtype=np.float32)
         fl = np.empty(nq, dtype=np.uint32)
-        o = make_opts(mode, slack, hash_log2, 0, retry)
+        o = make_opts(mode, slack, hash_log2, 0, retry, storage, rerank)
         _check(lib().song_search(self._h, _ptr(q), nq, pq, k, _ptr(ids), _ptr(ds), _ptr(fl), ctypes.byref(o)))
         out = ids.astype(np.int64)
         out[ids == np.uint64(0xFFFFFFFFFFFFFFFF)] = -1
         return out, ds, fl
 
     def search_device(self, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
-                      mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, stream=None):
+                      mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, stream=None, storage=STORE_F32, rerank=1,
+                      max_steps=0):
         """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k]."""
         import torch
 
@@ -137,7 +158,7 @@ class Index:
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
-        o = make_opts(mode, slack, hash_log2, 0, retry)
+        o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank)
         _check(lib().song_search_device(self._h, dp(queries), nq, pq, k, dp(pq_arr), dp(k_arr), dp(ids), dp(dists),
                                         ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
                                         ctypes.c_void_p(stream)))

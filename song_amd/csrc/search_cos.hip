@@ -2,7 +2,5 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_cos(bool exact, int shape) {
-    return exact ? pick_shape<M_COS, true>(shape) : pick_shape<M_COS, false>(shape);
-}
+SearchFn search_fn_cos(bool exact, int store, int shape) { return pick_kernel<M_COS>(exact, store, shape); }
 }  // namespace song

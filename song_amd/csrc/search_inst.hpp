@@ -1,5 +1,6 @@
-// Instantiation table for search_kernel<METRIC, EXACT, SHAPE>; SHAPE picks the lanes per
-// candidate row and the float4 registers per lane from the padded row width.
+// Instantiation table for search_kernel<METRIC, EXACT, STORE, SHAPE>.  SHAPE picks the
+// lanes per candidate row and the 16-byte units per lane from the units per row; EXACT
+// (double accumulation, bit-exact to the CPU) exists for float32 rows only.
 #pragma once
 
 #include "search_kernel.hpp"
@@ -8,31 +9,39 @@ namespace song {
 
 using SearchFn = void (*)(SearchParams);
 
-constexpr int kNumShapes = 5;
-
-// padded row width (floats) -> shape index
-inline int shape_for_ld(int ld) {
-    if (ld <= 32) return 0;
-    if (ld <= 128) return 1;
-    if (ld <= 256) return 2;
-    if (ld <= 1024) return 3;
+// 16-byte units per row -> shape index
+inline int shape_for_units(int units) {
+    if (units <= 8) return 0;
+    if (units <= 32) return 1;
+    if (units <= 64) return 2;
+    if (units <= 256) return 3;
     return 4;
 }
 
-template <int METRIC, bool EXACT>
+template <int METRIC, bool EXACT, int STORE>
 inline SearchFn pick_shape(int shape) {
     switch (shape) {
-    case 0: return search_kernel<METRIC, EXACT, 0>;
-    case 1: return search_kernel<METRIC, EXACT, 1>;
-    case 2: return search_kernel<METRIC, EXACT, 2>;
-    case 3: return search_kernel<METRIC, EXACT, 3>;
-    default: return search_kernel<METRIC, EXACT, 4>;
+    case 0: return search_kernel<METRIC, EXACT, STORE, 0>;
+    case 1: return search_kernel<METRIC, EXACT, STORE, 1>;
+    case 2: return search_kernel<METRIC, EXACT, STORE, 2>;
+    case 3: return search_kernel<METRIC, EXACT, STORE, 3>;
+    default: return search_kernel<METRIC, EXACT, STORE, 4>;
+    }
+}
+
+template <int METRIC>
+inline SearchFn pick_kernel(bool exact, int store, int shape) {
+    if (exact) return pick_shape<METRIC, true, ST_F32>(shape);
+    switch (store) {
+    case ST_BF16: return pick_shape<METRIC, false, ST_BF16>(shape);
+    case ST_FP8: return pick_shape<METRIC, false, ST_FP8>(shape);
+    default: return pick_shape<METRIC, false, ST_F32>(shape);
     }
 }
 
 // defined in search_{l2,ip,cos}.hip
-SearchFn search_fn_l2(bool exact, int shape);
-SearchFn search_fn_ip(bool exact, int shape);
-SearchFn search_fn_cos(bool exact, int shape);
+SearchFn search_fn_l2(bool exact, int store, int shape);
+SearchFn search_fn_ip(bool exact, int store, int shape);
+SearchFn search_fn_cos(bool exact, int store, int shape);
 
 }  // namespace song

@@ -2,7 +2,5 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_ip(bool exact, int shape) {
-    return exact ? pick_shape<M_IP, true>(shape) : pick_shape<M_IP, false>(shape);
-}
+SearchFn search_fn_ip(bool exact, int store, int shape) { return pick_kernel<M_IP>(exact, store, shape); }
 }  // namespace song

@@ -17,7 +17,7 @@
 //   queue    : qcap = pq + slack + 128 keys, sorted ascending, live range [h, h + nq), h < 64
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
 //   cid      : 64 u32    candidate ids of the current expansion
-//   qv       : ld f32    the query, zero-padded to ld
+//   qv       : qld f32   the query, zero-padded to a multiple of 16
 //
 // One wave per workgroup: LDS instructions of a wave execute in issue order, so lanes
 // exchange data through LDS with only a compiler fence between the write and the read
@@ -56,11 +56,16 @@ struct SearchParams {
     uint32_t *out_stats;    // nq x 4 or null: expansions, distances, sum(1+deg), prefetch hits
     uint64_t n;
     int dim, ld, gstride;
+    int qld;                // query floats in LDS: dim padded to 16
     int nq, pq, k, out_ld;
     int hlog, slack, max_steps, qcap;
     uint32_t *gvis;         // per-launch-slot spill tables (2^ghlog u32 each) or null
     int ghlog;
     int qbase;              // first query (or qidx entry) of this launch
+    const uint8_t *data_lp; // low-precision rows (BF16 / FP8 kernels), lp_stride bytes each
+    const float *scale;     // per-row scale of data_lp
+    size_t lp_stride;
+    int rerank;             // low-precision kernels: re-rank the final list in exact FP32
 };
 
 // ---------------------------------------------------------------- keys
@@ -217,60 +222,105 @@ __device__ double seq_sum(const float *row, const float *qv, int dim) {
     return s;
 }
 
+// ---------------------------------------------------------------- row storage
+
+// Base rows are read in 16-byte units.  ST_F32: the float32 rows (bfsg.data values,
+// zero-padded); ST_BF16 / ST_FP8: a low-precision copy, value = code * scale[row], with a
+// power-of-two scale for BF16 (lossless for integers up to 256) and max|x|/448 for FP8-E4M3.
+enum { ST_F32 = 0, ST_BF16 = 1, ST_FP8 = 2 };
+template <int STORE> struct Store;
+template <> struct Store<ST_F32> { static constexpr int EPU = 4; };
+template <> struct Store<ST_BF16> { static constexpr int EPU = 8; };
+template <> struct Store<ST_FP8> { static constexpr int EPU = 16; };
+
+template <int STORE>
+__device__ __forceinline__ void decode_unit(const uint4 u, float (&x)[Store<STORE>::EPU]) {
+    if constexpr (STORE == ST_F32) {
+        x[0] = __uint_as_float(u.x); x[1] = __uint_as_float(u.y);
+        x[2] = __uint_as_float(u.z); x[3] = __uint_as_float(u.w);
+    } else if constexpr (STORE == ST_BF16) {
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            x[2 * t] = __uint_as_float(w[t] << 16);
+            x[2 * t + 1] = __uint_as_float(w[t] & 0xffff0000u);
+        }
+    } else {
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[t], false);
+            const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[t], true);
+            x[4 * t] = lo[0]; x[4 * t + 1] = lo[1]; x[4 * t + 2] = hi[0]; x[4 * t + 3] = hi[1];
+        }
+    }
+}
+
 // Distances of the m candidates cid[0..m): returns lane c's key (c < m).
-// SUB lanes share one row; each lane keeps its share of all rows of a pass in
-// registers (up to VMAX float4) so every row load of the pass is issued before the first use.
-template <int METRIC, bool EXACT, int SUB, int VMAX, class Key>
+// SUB lanes share one row; each lane keeps its share of all rows of a pass in registers
+// (up to NVU 16-byte units) so every row load of the pass is issued before the first use.
+template <int METRIC, bool EXACT, int STORE, int SUB, int NVU, class Key>
 __device__ __forceinline__ Key eval_candidates(const SearchParams &P, int m, uint32_t cand, const uint32_t *cid,
                                                const float *qv, float lenv) {
+    constexpr int EPU = Store<STORE>::EPU;
     const int lane = threadIdx.x;
     constexpr int G = 64 / SUB;
     const int g = lane / SUB, sl = lane % SUB;
-    const int nv4 = P.ld >> 2;
-    const float4 *q4 = reinterpret_cast<const float4 *>(qv);
+    const int nu = (STORE == ST_F32) ? (P.ld >> 2) : (int)(P.lp_stride >> 4);
     Key mine = Key::inf();
     bool redo = false;
     for (int base = 0; base < m; base += G) {
         const int c = base + g;
         const bool valid = c < m;
         const uint32_t id = valid ? cid[c] : 0u;
-        const float4 *row = reinterpret_cast<const float4 *>(P.data + (size_t)id * (size_t)P.ld);
+        const uint4 *row = (STORE == ST_F32)
+                               ? reinterpret_cast<const uint4 *>(P.data + (size_t)id * (size_t)P.ld)
+                               : reinterpret_cast<const uint4 *>(P.data_lp + (size_t)id * P.lp_stride);
+        const float scale = (STORE != ST_F32 && valid) ? P.scale[id] : 1.0f;
         float facc = 0.0f;
         double dacc = 0.0, sab = 0.0;
         int mn = 0x3fffffff;
-        for (int j0 = 0; j0 < nv4; j0 += VMAX * SUB) {
-            float4 x[VMAX];
+        for (int j0 = 0; j0 < nu; j0 += NVU * SUB) {
+            uint4 u[NVU];
 #pragma unroll
-            for (int v = 0; v < VMAX; ++v) {
+            for (int v = 0; v < NVU; ++v) {
                 const int j = j0 + sl + v * SUB;
-                if (valid && j < nv4) x[v] = row[j];
+                if (valid && j < nu) u[v] = row[j];
             }
 #pragma unroll
-            for (int v = 0; v < VMAX; ++v) {
+            for (int v = 0; v < NVU; ++v) {
                 const int j = j0 + sl + v * SUB;
-                if (valid && j < nv4) {
-                    const float4 q = q4[j];
+                if (valid && j < nu) {
+                    float x[EPU];
+                    decode_unit<STORE>(u[v], x);
+                    const float *q = qv + j * EPU;
                     if constexpr (!EXACT) {
-                        if (METRIC == M_L2) {
-                            float a = x[v].x - q.x, b = x[v].y - q.y, cc = x[v].z - q.z, d = x[v].w - q.w;
-                            facc += a * a + b * b + cc * cc + d * d;
-                        } else {
-                            facc += x[v].x * q.x + x[v].y * q.y + x[v].z * q.z + x[v].w * q.w;
+#pragma unroll
+                        for (int t = 0; t < EPU; ++t) {
+                            const float xv = (STORE == ST_F32) ? x[t] : x[t] * scale;
+                            if (METRIC == M_L2) {
+                                const float d = xv - q[t];
+                                facc += d * d;
+                            } else {
+                                facc += xv * q[t];
+                            }
                         }
                     } else {
                         // float products (rounded), double accumulation; exact in any order when
                         // all products are multiples of 2^mn and sum |p| < 2^(mn+53)
-                        float p0, p1, p2, p3;
-                        if (METRIC == M_L2) {
-                            float a = x[v].x - q.x, b = x[v].y - q.y, cc = x[v].z - q.z, d = x[v].w - q.w;
-                            p0 = a * a; p1 = b * b; p2 = cc * cc; p3 = d * d;
-                        } else {
-                            p0 = x[v].x * q.x; p1 = x[v].y * q.y; p2 = x[v].z * q.z; p3 = x[v].w * q.w;
+#pragma unroll
+                        for (int t = 0; t < EPU; ++t) {
+                            float pr;
+                            if (METRIC == M_L2) {
+                                const float d = x[t] - q[t];
+                                pr = d * d;
+                            } else {
+                                pr = x[t] * q[t];
+                            }
+                            dacc += (double)pr;
+                            sab += (double)fabsf(pr);
+                            mn = min(mn, lsb_exp(pr));
                         }
-                        dacc += (double)p0; dacc += (double)p1; dacc += (double)p2; dacc += (double)p3;
-                        sab += (double)fabsf(p0); sab += (double)fabsf(p1);
-                        sab += (double)fabsf(p2); sab += (double)fabsf(p3);
-                        mn = min(mn, min(min(lsb_exp(p0), lsb_exp(p1)), min(lsb_exp(p2), lsb_exp(p3))));
                     }
                 }
             }
@@ -320,7 +370,16 @@ __device__ __forceinline__ Key eval_candidates(const SearchParams &P, int m, uin
     return mine;
 }
 
-// ---------------------------------------------------------------- the search kernel
+// wave-wide minimum of a KeyF (all lanes get it)
+__device__ __forceinline__ KeyF wave_min(KeyF k) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t v = ((uint64_t)(uint32_t)__shfl_xor((int)(k.v >> 32), o) << 32) |
+                           (uint32_t)__shfl_xor((int)k.v, o);
+        if (v < k.v) k.v = v;
+    }
+    return k;
+}
 
 // Count of the leading entries of the sorted run arr[0..n) that satisfy `below`, a
 // predicate that holds on a prefix.  64-ary search: each round samples the last entry of
@@ -344,17 +403,17 @@ __device__ __forceinline__ int prefix_count(const Key *arr, int n, Pred below) {
     return lo + __popcll(ballot(v && below(e)));
 }
 
-// shapes: (lanes per candidate row, float4 registers per lane) by padded row width
+// shapes: (lanes per candidate row, 16-byte units per lane) by units per row
 template <int SHAPE> struct Shape;
-template <> struct Shape<0> { static constexpr int SUB = 4, NV = 2; };    // ld <= 32
-template <> struct Shape<1> { static constexpr int SUB = 4, NV = 8; };    // ld <= 128
-template <> struct Shape<2> { static constexpr int SUB = 4, NV = 16; };   // ld <= 256
-template <> struct Shape<3> { static constexpr int SUB = 16, NV = 16; };  // ld <= 1024
+template <> struct Shape<0> { static constexpr int SUB = 4, NV = 2; };    // <= 8 units
+template <> struct Shape<1> { static constexpr int SUB = 4, NV = 8; };    // <= 32 units
+template <> struct Shape<2> { static constexpr int SUB = 4, NV = 16; };   // <= 64 units
+template <> struct Shape<3> { static constexpr int SUB = 16, NV = 16; };  // <= 256 units
 template <> struct Shape<4> { static constexpr int SUB = 64, NV = 16; };  // wider rows, chunked
 
 // ---------------------------------------------------------------- the search kernel
 
-template <int METRIC, bool EXACT, int SHAPE>
+template <int METRIC, bool EXACT, int STORE, int SHAPE>
 __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     using Key = typename KeySel<EXACT>::T;
     constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
@@ -377,7 +436,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     float *qv = reinterpret_cast<float *>(cid + 64);
 
     for (uint32_t i = lane; i < H; i += 64) vis[i] = 0u;
-    for (int i = lane; i < P.ld; i += 64) qv[i] = (i < P.dim) ? P.queries[(size_t)qi * P.dim + i] : 0.0f;
+    for (int i = lane; i < P.qld; i += 64) qv[i] = (i < P.dim) ? P.queries[(size_t)qi * P.dim + i] : 0.0f;
     lds_fence();
 
     float lenv = 0.0f;
@@ -408,7 +467,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         }
         nvis = 1u;
         lds_fence();
-        const Key k0 = eval_candidates<METRIC, EXACT, SUB, NV, Key>(P, 1, start, cid, qv, lenv);
+        const Key k0 = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, 1, start, cid, qv, lenv);
         if (lane == 0) Q[0] = k0;
         nq = 1;
         lds_fence();
@@ -497,7 +556,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         if (m == 0) continue;
         lds_fence();
         const uint32_t cand = (lane < m) ? cid[lane] : 0u;
-        const Key mine = eval_candidates<METRIC, EXACT, SUB, NV, Key>(P, m, cand, cid, qv, lenv);
+        const Key mine = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, m, cand, cid, qv, lenv);
 
         // ---- merge the candidates into the queue
         uint64_t cut = (nbest == pq) ? bmax : ~0ull;  // drop distance keys above cut
@@ -561,10 +620,45 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         }
     }
 
-    // output: first k of the ascending list (graph.h:209-215)
     const int cnt = min(k, nbest);
     const size_t ob = (size_t)qi * (size_t)P.out_ld;
-    for (int i = lane; i < k; i += 64) {
+    bool done = false;
+    if constexpr (STORE != ST_F32 && !EXACT) {
+        if (P.rerank) {
+            // exact FP32 distances for the whole pq-best list, then the k smallest in order
+            for (int base = 0; base < nbest; base += 64) {
+                const int c = min(64, nbest - base);
+                if (lane < c) cid[lane] = B[base + lane].id();
+                lds_fence();
+                const uint32_t cand = (lane < c) ? cid[lane] : 0u;
+                const Key ex = eval_candidates<METRIC, false, ST_F32, 4, 8, Key>(P, c, cand, cid, qv, lenv);
+                if (lane < c) B[base + lane] = ex;
+                lds_fence();
+            }
+            for (int r = 0; r < k; ++r) {
+                Key best = Key::inf();
+                int bi = -1;
+                for (int i = lane; i < nbest; i += 64) {
+                    const Key e = B[i];
+                    if (lt(e, best)) {
+                        best = e;
+                        bi = i;
+                    }
+                }
+                const Key w = wave_min(best);
+                if (bi >= 0 && best.v == w.v) B[bi] = Key::inf();
+                lds_fence();
+                if (lane == 0) {
+                    const bool ok = r < cnt;
+                    P.out_ids[ob + r] = ok ? w.id() : 0xFFFFFFFFu;
+                    if (P.out_dist) P.out_dist[ob + r] = ok ? w.dist() : __int_as_float(0x7f800000);
+                }
+            }
+            done = true;
+        }
+    }
+    // output: first k of the ascending list (graph.h:209-215)
+    for (int i = lane; i < k && !done; i += 64) {
         const bool ok = i < cnt;
         const Key e = ok ? B[i] : Key::inf();
         P.out_ids[ob + i] = ok ? e.id() : 0xFFFFFFFFu;

@@ -2,7 +2,5 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_l2(bool exact, int shape) {
-    return exact ? pick_shape<M_L2, true>(shape) : pick_shape<M_L2, false>(shape);
-}
+SearchFn search_fn_l2(bool exact, int store, int shape) { return pick_kernel<M_L2>(exact, store, shape); }
 }  // namespace song

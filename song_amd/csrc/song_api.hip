@@ -46,6 +46,9 @@ struct song_index {
     size_t max_lds = 65536;
     uint32_t *d_gvis = nullptr;  // visited spill tables, one per launch slot
     size_t gvis_slots = 0;
+    uint8_t *d_lp[3] = {nullptr, nullptr, nullptr};  // low-precision rows by SONG_STORE_*
+    float *d_scale[3] = {nullptr, nullptr, nullptr};
+    size_t lp_stride[3] = {0, 0, 0};
 };
 
 // visited spill tables: 2^13 u32 (32 KB) per launch slot, launches of at most kChunk queries
@@ -70,6 +73,54 @@ extern "C" void song_search_opts_default(song_search_opts *o) {
     o->hash_log2 = 0;
     o->max_steps = 0;
     o->retry = 1;
+    o->storage = SONG_STORE_F32;
+    o->rerank = 1;
+}
+
+// Low-precision copy of one row per thread: BF16 with a power-of-two scale (round to
+// nearest even), or FP8-E4M3 with scale max|x|/448 (hardware conversion of the clamped
+// value).  The scale of an all-zero row is 1.
+__global__ void quantize_kernel(const float *data, uint64_t n, int dim, int ld, int store, uint8_t *out,
+                                size_t stride, float *scale) {
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float *r = data + v * (uint64_t)ld;
+    float amax = 0.0f;
+    for (int i = 0; i < dim; ++i) amax = fmaxf(amax, fabsf(r[i]));
+    float s = 1.0f;
+    if (amax > 0.0f) {
+        if (store == SONG_STORE_BF16) {
+            int e;
+            frexpf(amax, &e);
+            s = ldexpf(1.0f, e);
+        } else {
+            s = amax / 448.0f;
+        }
+    }
+    scale[v] = s;
+    uint32_t *o = reinterpret_cast<uint32_t *>(out + v * stride);
+    const int words = (int)(stride / 4);
+    for (int w = 0; w < words; ++w) {
+        uint32_t word = 0u;
+        if (store == SONG_STORE_BF16) {
+            for (int t = 0; t < 2; ++t) {
+                const int i = 2 * w + t;
+                const float x = i < dim ? r[i] / s : 0.0f;
+                const uint32_t u = __float_as_uint(x);
+                const uint32_t code = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+                word |= (code & 0xffffu) << (16 * t);
+            }
+        } else {
+            float y[4];
+            for (int t = 0; t < 4; ++t) {
+                const int i = 4 * w + t;
+                y[t] = i < dim ? fminf(fmaxf(r[i] / s, -448.0f), 448.0f) : 0.0f;
+            }
+            int lo = __builtin_amdgcn_cvt_pk_fp8_f32(y[0], y[1], 0, false);
+            word = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(y[2], y[3], lo, true);
+        }
+        o[w] = word;
+    }
 }
 
 // squared norm per base row with one sequential fmaf chain (data.h:54 as the reference
@@ -195,8 +246,39 @@ extern "C" int song_index_create(const float *data, uint64_t n, int dim, const u
     return SONG_OK;
 }
 
+extern "C" int song_index_add_storage(song_index *ix, int store) {
+    if (!ix) return fail(SONG_E_ARG, "index is null");
+    if (store != SONG_STORE_BF16 && store != SONG_STORE_FP8) return fail(SONG_E_ARG, "storage=%d", store);
+    if (ix->d_lp[store]) return SONG_OK;
+    const int epu = store == SONG_STORE_BF16 ? 8 : 16;
+    const size_t esz = store == SONG_STORE_BF16 ? 2 : 1;
+    const size_t stride = (size_t)((ix->dim + epu - 1) / epu) * epu * esz;
+    HIP_TRY(hipMalloc(&ix->d_lp[store], stride * ix->n));
+    HIP_TRY(hipMalloc(&ix->d_scale[store], sizeof(float) * ix->n));
+    hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)((ix->n + 255) / 256)), dim3(256), 0, 0, ix->d_data, ix->n,
+                       ix->dim, ix->ld, store, ix->d_lp[store], stride, ix->d_scale[store]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    ix->lp_stride[store] = stride;
+    return SONG_OK;
+}
+
+extern "C" int song_index_get_storage(const song_index *ix, int store, void *codes, float *scales,
+                                      uint64_t *row_bytes) {
+    if (!ix || store < SONG_STORE_BF16 || store > SONG_STORE_FP8 || !ix->d_lp[store])
+        return fail(SONG_E_ARG, "storage %d not built", store);
+    if (row_bytes) *row_bytes = ix->lp_stride[store];
+    if (codes) HIP_TRY(hipMemcpy(codes, ix->d_lp[store], ix->lp_stride[store] * ix->n, hipMemcpyDeviceToHost));
+    if (scales) HIP_TRY(hipMemcpy(scales, ix->d_scale[store], sizeof(float) * ix->n, hipMemcpyDeviceToHost));
+    return SONG_OK;
+}
+
 extern "C" int song_index_destroy(song_index *ix) {
     if (!ix) return SONG_OK;
+    for (int i = 0; i < 3; ++i) {
+        if (ix->d_lp[i]) hipFree(ix->d_lp[i]);
+        if (ix->d_scale[i]) hipFree(ix->d_scale[i]);
+    }
     if (ix->d_data) hipFree(ix->d_data);
     if (ix->d_graph) hipFree(ix->d_graph);
     if (ix->d_norms) hipFree(ix->d_norms);
@@ -211,24 +293,28 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 struct Plan {
     bool exact;
-    int shape, hlog, slack, qcap;
+    int store, shape, hlog, slack, qcap, qld;
     size_t lds;
 };
 
-static size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int ld) {
+static size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld) {
     size_t key = exact ? 16 : 8;
-    return ((size_t)4 << hlog) + (size_t)qcap * key + (size_t)((pq + 1) & ~1) * key + 64 * 4 + (size_t)ld * 4;
+    return ((size_t)4 << hlog) + (size_t)qcap * key + (size_t)((pq + 1) & ~1) * key + 64 * 4 + (size_t)qld * 4;
 }
 
-static int make_plan(const song_index *ix, bool exact, int pq, int slack, int hlog, Plan *p) {
+static int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
-    p->exact = exact;
-    p->shape = shape_for_ld(ix->ld);
+    if (store != SONG_STORE_F32 && !ix->d_lp[store]) return fail(SONG_E_ARG, "storage %d not built", store);
+    p->exact = exact && store == SONG_STORE_F32;
+    p->store = store;
+    const int units = store == SONG_STORE_F32 ? ix->ld / 4 : (int)(ix->lp_stride[store] / 16);
+    p->shape = shape_for_units(units);
     p->hlog = hlog;
     p->slack = slack;
     // queue buffer: pq + slack live entries, 64 of merge room, 64 of head room
     p->qcap = pq + slack + 128;
-    p->lds = lds_bytes(exact, pq, p->qcap, hlog, ix->ld);
+    p->qld = (ix->dim + 15) & ~15;
+    p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld);
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
     return SONG_OK;
@@ -241,9 +327,9 @@ static int default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
 static SearchFn search_fn(int metric, const Plan &p) {
     switch (metric) {
-    case SONG_L2: return search_fn_l2(p.exact, p.shape);
-    case SONG_IP: return search_fn_ip(p.exact, p.shape);
-    default: return search_fn_cos(p.exact, p.shape);
+    case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape);
+    case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape);
+    default: return search_fn_cos(p.exact, p.store, p.shape);
     }
 }
 
@@ -254,6 +340,10 @@ static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned
     sp.qcap = p.qcap;
     sp.gvis = ix->d_gvis;
     sp.ghlog = kGhlog;
+    sp.qld = p.qld;
+    sp.data_lp = ix->d_lp[p.store];
+    sp.scale = ix->d_scale[p.store];
+    sp.lp_stride = ix->lp_stride[p.store];
     if (p.lds > 65536)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
     for (unsigned q0 = 0; q0 < count; q0 += kChunk) {
@@ -295,8 +385,10 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
     if (opts.max_steps <= 0) opts.max_steps = 1000000;
     hipStream_t st = (hipStream_t)stream;
 
+    if (opts.storage < SONG_STORE_F32 || opts.storage > SONG_STORE_FP8)
+        return fail(SONG_E_ARG, "storage=%d", opts.storage);
     bool exact = true;
-    if (opts.mode == SONG_MODE_FP32) {
+    if (opts.storage != SONG_STORE_F32 || opts.mode == SONG_MODE_FP32) {
         exact = false;
     } else if (opts.mode == SONG_MODE_AUTO && ix->integer_valued && ix->metric != SONG_COS) {
         int qint = 0;
@@ -311,7 +403,7 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
     int hlog = opts.hash_log2 > 0 ? opts.hash_log2 : default_hlog(pq);
     hlog = std::max(9, std::min(hlog, 16));
     Plan plan;
-    int rc = make_plan(ix, exact, pq, opts.slack, hlog, &plan);
+    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan);
     if (rc != SONG_OK) return rc;
 
     uint32_t *flags = d_flags;
@@ -342,6 +434,7 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
     sp.k = k;
     sp.out_ld = k_ld;
     sp.max_steps = opts.max_steps;
+    sp.rerank = opts.rerank;
     rc = launch(ix, plan, sp, (unsigned)nq, st);
     if (rc == SONG_OK && opts.retry) {
         // re-run queries whose LDS tables overflowed, with 4x larger tables, until none remain
@@ -369,7 +462,7 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
             Plan p2;
             int h2 = vis ? plan.hlog + 2 : plan.hlog;
             int s2 = que ? plan.slack * 4 + 64 : plan.slack;
-            rc = make_plan(ix, exact, pq, s2, std::min(h2, 16), &p2);
+            rc = make_plan(ix, exact, opts.storage, pq, s2, std::min(h2, 16), &p2);
             if (rc != SONG_OK) break;
             plan = p2;
             if (!d_idx && hipMalloc(&d_idx, nq * sizeof(uint32_t)) != hipSuccess) {
