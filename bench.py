@@ -100,7 +100,7 @@ def main():
         ix.search_device(q, pq, k, ids, None, flags, stats, retry=retry)
 
     flags = torch.empty(nq, dtype=torch.int32, device=dev)
-    stats = torch.empty((nq, 4), dtype=torch.int32, device=dev)
+    stats = torch.empty((nq, 8), dtype=torch.int32, device=dev)
     ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
 
     sweep = []

@@ -116,9 +116,11 @@ int song_search(song_index *index, const float *queries, uint64_t nq, int pq, in
  * Batched search with device buffers on `stream` (a hipStream_t, NULL = default).
  * d_queries nq x dim float32; d_pq / d_k: per-query budgets (nullable: use pq
  * and k for all); d_ids nq x k_ld uint32 (0xFFFFFFFF padding); d_dist nq x k_ld
- * float32 (nullable); d_flags nq uint32 (nullable); d_stats nq x 4 uint32
+ * float32 (nullable); d_flags nq uint32 (nullable); d_stats nq x 8 uint32
  * (expansions, distances computed, adjacency words read = sum of 1 + degree
- * over expanded vertices, 0; nullable).  pq is the maximum budget.
+ * over expanded vertices, adjacency-row prefetch hits, settle step = the last
+ * expansion that changed the first k results, 0, 0, 0; nullable).  pq is the
+ * maximum budget.
  * With opts->retry set (the default) the call reads the flags back, re-runs
  * overflowed queries and synchronizes the stream; with retry = 0 it only
  * enqueues the search and the caller checks d_flags.
@@ -126,6 +128,26 @@ int song_search(song_index *index, const float *queries, uint64_t nq, int pq, in
 int song_search_device(song_index *index, const float *d_queries, uint64_t nq, int pq, int k,
                        const int *d_pq, const int *d_k, uint32_t *d_ids, float *d_dist, int k_ld,
                        uint32_t *d_flags, uint32_t *d_stats, const song_search_opts *opts, void *stream);
+
+/*
+ * Per-query parameters of song_search_device_ex (all nullable).  No counterpart in the
+ * reference, which searches every query with the compiled TOPK from vertex 0
+ * (warp_astar_accelerator.h:166-176, 386); used by the learned and batching forms.
+ */
+typedef struct {
+    const int *pq;          /* nq budgets (each <= the call's pq) */
+    const int *k;           /* nq result counts (each <= its pq and <= k_ld) */
+    const uint32_t *entry;  /* nq x n_entry start vertices, in place of vertex 0 */
+    int n_entry;            /* 1..64 */
+    const int *steps;       /* nq expansion caps */
+    const uint32_t *subset; /* search only these query indices (results land at those rows) */
+    uint64_t n_subset;
+} song_query_params;
+
+/* song_search_device with per-query parameters (qp nullable = the plain call). */
+int song_search_device_ex(song_index *index, const float *d_queries, uint64_t nq, int pq, int k,
+                          const song_query_params *qp, uint32_t *d_ids, float *d_dist, int k_ld, uint32_t *d_flags,
+                          uint32_t *d_stats, const song_search_opts *opts, void *stream);
 
 /*
  * One-shot replacement of WarpAStarAccelerator::astar_multi_start_search_batch

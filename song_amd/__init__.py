@@ -28,6 +28,11 @@ class SongError(RuntimeError):
     pass
 
 
+class QueryParams(ctypes.Structure):
+    _fields_ = [("pq", ctypes.c_void_p), ("k", ctypes.c_void_p), ("entry", ctypes.c_void_p), ("n_entry", ctypes.c_int),
+                ("steps", ctypes.c_void_p), ("subset", ctypes.c_void_p), ("n_subset", ctypes.c_uint64)]
+
+
 class SearchOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slack", ctypes.c_int), ("hash_log2", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("retry", ctypes.c_int), ("storage", ctypes.c_int),
@@ -58,6 +63,8 @@ def lib():
         L.song_search_device.argtypes = [vp, vp, u64, i32, i32, vp, vp, vp, vp, i32, vp, vp,
                                          ctypes.POINTER(SearchOpts), vp]
         L.song_astar_multi_start_search_batch.argtypes = [vp, u64, i32, vp, vp, vp, i32, u64, i32, i32]
+        L.song_search_device_ex.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(QueryParams), vp, vp, i32, vp,
+                                            vp, ctypes.POINTER(SearchOpts), vp]
         L.song_index_add_storage.argtypes = [vp, i32]
         L.song_index_get_storage.argtypes = [vp, i32, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
@@ -67,7 +74,7 @@ def lib():
 EXPORTED_SYMBOLS = (
     "song_search_opts_default", "song_index_create", "song_index_destroy", "song_index_integer_valued",
     "song_search", "song_search_device", "song_astar_multi_start_search_batch", "song_last_error",
-    "song_index_add_storage", "song_index_get_storage",
+    "song_index_add_storage", "song_index_get_storage", "song_search_device_ex",
 )
 
 
@@ -150,8 +157,12 @@ class Index:
 
     def search_device(self, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
                       mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, stream=None, storage=STORE_F32, rerank=1,
-                      max_steps=0):
-        """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k]."""
+                      max_steps=0, entry=None, steps=None, subset=None):
+        """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k].
+
+        entry: int32 [nq] or [nq, E] start vertices (learned form; default vertex 0);
+        steps: int32 [nq] per-query expansion caps; subset: int32 query indices to run;
+        stats: int32 [nq, 8]."""
         import torch
 
         nq = queries.shape[0]
@@ -159,9 +170,16 @@ class Index:
             stream = torch.cuda.current_stream().cuda_stream
         dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
         o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank)
-        _check(lib().song_search_device(self._h, dp(queries), nq, pq, k, dp(pq_arr), dp(k_arr), dp(ids), dp(dists),
-                                        ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
-                                        ctypes.c_void_p(stream)))
+        qp = QueryParams()
+        qp.pq, qp.k = (pq_arr.data_ptr() if pq_arr is not None else None), (k_arr.data_ptr() if k_arr is not None else None)
+        qp.entry = entry.data_ptr() if entry is not None else None
+        qp.n_entry = 1 if entry is None else (entry.shape[1] if entry.dim() == 2 else 1)
+        qp.steps = steps.data_ptr() if steps is not None else None
+        qp.subset = subset.data_ptr() if subset is not None else None
+        qp.n_subset = subset.numel() if subset is not None else 0
+        _check(lib().song_search_device_ex(self._h, dp(queries), nq, pq, k, ctypes.byref(qp), dp(ids), dp(dists),
+                                           ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
+                                           ctypes.c_void_p(stream)))
 
 
 # ---------------------------------------------------------------- host library (CPU side)

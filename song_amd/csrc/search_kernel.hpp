@@ -46,14 +46,17 @@ struct SearchParams {
     const uint32_t *graph;  // n x gstride, [deg, ids...]
     const float *norms;     // cosine: per-vertex squared norm (fmaf chain), else null
     const float *queries;   // nq x dim
-    const uint32_t *entry;  // per-query start vertex or null (vertex 0)
+    const uint32_t *entry;  // nq x n_entry start vertices or null (vertex 0)
+    int n_entry;            // entries per query (1..64)
+    const int *steps_arr;   // per-query expansion cap or null
     const int *pq_arr;      // per-query pq or null
     const int *k_arr;       // per-query k or null
     const uint32_t *qidx;   // query index remap (retry pass) or null
     uint32_t *out_ids;      // nq x out_ld
     float *out_dist;        // nq x out_ld or null
     uint32_t *out_flags;    // nq or null
-    uint32_t *out_stats;    // nq x 4 or null: expansions, distances, sum(1+deg), prefetch hits
+    uint32_t *out_stats;    // nq x 8 or null: expansions, distances, sum(1+deg), prefetch hits,
+                            // settle step (last expansion that changed the first k), 0, 0, 0
     uint64_t n;
     int dim, ld, gstride;
     int qld;                // query floats in LDS: dim padded to 16
@@ -458,26 +461,33 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     int h = 0, nq = 0, nbest = 0;
     uint64_t bmax = ~0ull;  // distance key of best[pq-1] once the list is full
 
-    // start vertex (graph.h:175-181)
-    const uint32_t start = P.entry ? P.entry[qi] : 0u;
+    // start vertices (graph.h:175-181: vertex 0; the learned form passes its own), each
+    // entered once through the visited set and placed in the queue in (dist, id) order
     {
-        if (lane == 0) {
-            vis_insert(vis, hmask, P.hlog, start);
-            cid[0] = start;
-        }
-        nvis = 1u;
+        const int ne = P.entry ? P.n_entry : 1;
+        const uint32_t sid = (lane < ne) ? (P.entry ? P.entry[(size_t)qi * ne + lane] : 0u) : 0u;
+        const bool fresh = lane < ne && vis_insert(vis, hmask, P.hlog, sid);
+        const uint64_t fm = ballot(fresh);
+        const int m0 = __popcll(fm);
+        if (fresh) cid[lanes_below(fm)] = sid;
+        nvis = (uint32_t)m0;
         lds_fence();
-        const Key k0 = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, 1, start, cid, qv, lenv);
-        if (lane == 0) Q[0] = k0;
-        nq = 1;
+        const uint32_t cand0 = (lane < m0) ? cid[lane] : 0u;
+        const Key k0 = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, m0, cand0, cid, qv, lenv);
+        int rank = 0;
+        for (int j = 0; j < m0; ++j) rank += lt(k0.rl(j), k0) ? 1 : 0;
+        if (lane < m0) Q[rank] = k0;
+        nq = m0;
         lds_fence();
     }
+    const int max_steps = P.steps_arr ? min(P.steps_arr[qi], P.max_steps) : P.max_steps;
+    int settle = 0;
 
     uint32_t pf_id = 0xFFFFFFFFu;  // vertex whose adjacency row sits in pf_row
     uint32_t pf_row = 0u;
 
     while (nq > 0) {
-        if (steps >= P.max_steps) {
+        if (steps >= max_steps) {
             flags |= F_STEP_CAP;
             break;
         }
@@ -508,6 +518,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         // find its position, shift the entries behind it one place (last block first)
         {
             const int p = prefix_count(B, nbest, [&](const Key &e) { return lt(e, now); });
+            if (p < k) settle = steps;
             if (p < pq) {
                 const int last = min(nbest, pq - 1);  // entries [p, last) move to [p+1, last+1)
                 for (int blk = (last - 1) >> 6; blk >= (p >> 6) && last > p; --blk) {
@@ -667,10 +678,15 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     if (lane == 0) {
         if (P.out_flags) P.out_flags[qi] = flags;
         if (P.out_stats) {
-            P.out_stats[4 * (size_t)qi] = (uint32_t)steps;
-            P.out_stats[4 * (size_t)qi + 1] = nvis;
-            P.out_stats[4 * (size_t)qi + 2] = adj_words;
-            P.out_stats[4 * (size_t)qi + 3] = pf_hits;
+            uint32_t *o = P.out_stats + 8 * (size_t)qi;
+            o[0] = (uint32_t)steps;
+            o[1] = nvis;
+            o[2] = adj_words;
+            o[3] = pf_hits;
+            o[4] = (uint32_t)settle;
+            o[5] = 0u;
+            o[6] = 0u;
+            o[7] = 0u;
         }
     }
 }

@@ -373,6 +373,26 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
                                   const int *d_pq, const int *d_k, uint32_t *d_ids, float *d_dist, int k_ld,
                                   uint32_t *d_flags, uint32_t *d_stats, const song_search_opts *opts_in,
                                   void *stream) {
+    song_query_params qp{};
+    qp.pq = d_pq;
+    qp.k = d_k;
+    qp.n_entry = 1;
+    return song_search_device_ex(ix, d_queries, nq, pq, k, &qp, d_ids, d_dist, k_ld, d_flags, d_stats, opts_in,
+                                 stream);
+}
+
+extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uint64_t nq, int pq, int k,
+                                     const song_query_params *qp_in, uint32_t *d_ids, float *d_dist, int k_ld,
+                                     uint32_t *d_flags, uint32_t *d_stats, const song_search_opts *opts_in,
+                                     void *stream) {
+    song_query_params qp{};
+    qp.n_entry = 1;
+    if (qp_in) qp = *qp_in;
+    const int *d_pq = qp.pq, *d_k = qp.k;
+    const uint32_t *d_entry = qp.entry;
+    const int n_entry = qp.entry ? qp.n_entry : 1, *d_steps = qp.steps;
+    if (n_entry < 1 || n_entry > 64) return fail(SONG_E_ARG, "n_entry=%d not in [1,64]", n_entry);
+    const uint64_t nrun = qp.subset ? qp.n_subset : nq;
     if (!ix) return fail(SONG_E_ARG, "index is null");
     if (nq == 0) return SONG_OK;
     if (!d_queries || !d_ids) return fail(SONG_E_ARG, "queries/ids are null");
@@ -417,10 +437,12 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
     sp.graph = ix->d_graph;
     sp.norms = ix->d_norms;
     sp.queries = d_queries;
-    sp.entry = nullptr;
+    sp.entry = d_entry;
+    sp.n_entry = n_entry;
+    sp.steps_arr = d_steps;
     sp.pq_arr = d_pq;
     sp.k_arr = d_k;
-    sp.qidx = nullptr;
+    sp.qidx = qp.subset;
     sp.out_ids = d_ids;
     sp.out_dist = d_dist;
     sp.out_flags = flags;
@@ -435,10 +457,20 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
     sp.out_ld = k_ld;
     sp.max_steps = opts.max_steps;
     sp.rerank = opts.rerank;
-    rc = launch(ix, plan, sp, (unsigned)nq, st);
+    if (nrun == 0) {
+        if (tmp_flags) hipFree(tmp_flags);
+        return SONG_OK;
+    }
+    rc = launch(ix, plan, sp, (unsigned)nrun, st);
     if (rc == SONG_OK && opts.retry) {
         // re-run queries whose LDS tables overflowed, with 4x larger tables, until none remain
         std::vector<uint32_t> hf(nq);
+        std::vector<uint32_t> run_ids;
+        if (qp.subset) {
+            run_ids.resize(nrun);
+            if (hipMemcpyAsync(run_ids.data(), qp.subset, nrun * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+                rc = fail(SONG_E_HIP, "subset readback");
+        }
         uint32_t *d_idx = nullptr;
         for (int round = 0; rc == SONG_OK; ++round) {
             if (hipMemcpyAsync(hf.data(), flags, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -448,12 +480,14 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
             }
             std::vector<uint32_t> redo;
             bool vis = false, que = false;
-            for (uint64_t i = 0; i < nq; ++i)
+            for (uint64_t j = 0; j < nrun; ++j) {
+                const uint64_t i = run_ids.empty() ? j : run_ids[j];
                 if (hf[i] & (SONG_FLAG_VISITED_OVF | SONG_FLAG_QUEUE_OVF)) {
                     redo.push_back((uint32_t)i);
                     vis |= (hf[i] & SONG_FLAG_VISITED_OVF) != 0;
                     que |= (hf[i] & SONG_FLAG_QUEUE_OVF) != 0;
                 }
+            }
             if (redo.empty()) break;
             if (round == 4) {
                 rc = fail(SONG_E_OVERFLOW, "%zu queries still overflow after 4 re-runs", redo.size());
@@ -476,6 +510,7 @@ extern "C" int song_search_device(song_index *ix, const float *d_queries, uint64
             SearchParams sp2 = sp;
             sp2.qidx = d_idx;
             rc = launch(ix, plan, sp2, (unsigned)redo.size(), st);
+            if (rc != SONG_OK) break;
         }
         if (d_idx) hipFree(d_idx);
     }

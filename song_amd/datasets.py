@@ -61,6 +61,16 @@ def mixture_l2(n_base: int = 1_000_000, n_query: int = 10_000, dim: int = 128,
     return base, query
 
 
+def mixture_l2_queries(n: int, dim: int = 128, n_centers: int = 1000, sigma: float = 20.0, seed: int = 2,
+                       stream: int = 1):
+    """More queries from the same mixture as mixture_l2(seed=seed) (same centres, an
+    independent stream), e.g. training queries for the learned form."""
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(0.0, 255.0, size=(n_centers, dim)).astype(np.float32)
+    rq = np.random.default_rng([seed, 1000 + stream])
+    return _mixture_rows(rq, centers, n, sigma)
+
+
 def factor_l2(n_base: int = 1_000_000, n_query: int = 1000, dim: int = 960, rank: int = 64,
               noise: float = 0.05, seed: int = 3, chunk: int = 1 << 16):
     """X = Z W^T + N(0, noise^2), then one global min-max scaling to [0,1]."""

@@ -1,0 +1,300 @@
+"""Learned form: an MLP that predicts, per query, start vertices and a stop step.
+
+No counterpart in the reference, whose search always starts at vertex 0
+(graph.h:171-181, warp_astar_accelerator.h:166-176) and always runs to its
+pq-based stop rule (graph.h:188).
+
+Pipeline (all heavy work on the GPU):
+  pivots   k-means (n_pivots centroids) over a sample of the base rows; each
+           centroid maps to its nearest base vertex, the pivot's start vertex.
+  traces   the plain search (from vertex 0) of the training queries; the label
+           of a query is the centroid nearest to its best result, and its stop
+           target is log2(1 + settle step) of a search started at the label's
+           two nearest pivots (settle step = the last expansion that changed
+           the top-k, reported by the search kernel).
+  network  song_learned.h: dim + 8 features -> hidden -> hidden -> n_pivots + 1,
+           BF16 MFMA forward/backward kernels, Adam.
+  search   confident queries (softmax mass of the two best pivots >= threshold)
+           start at those two pivot vertices with budget pq and an expansion
+           cap from the stop head; the rest fall back to the plain full-budget
+           search from vertex 0.  Both groups run in one call each through
+           song_search_device_ex with a query subset.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import SongError, lib, _check
+
+
+def _mlp_lib():
+    L = lib()
+    if not hasattr(L, "_mlp_bound"):
+        vp, i32, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+        L.song_mlp_last_error.restype = ctypes.c_char_p
+        L.song_mlp_create.argtypes = [i32, i32, i32, i32, u64, ctypes.POINTER(vp)]
+        L.song_mlp_destroy.argtypes = [vp]
+        L.song_mlp_input_width.argtypes = [vp]
+        L.song_mlp_param_count.argtypes = [vp, ctypes.POINTER(u64)]
+        L.song_mlp_get_params.argtypes = [vp, vp]
+        L.song_mlp_set_params.argtypes = [vp, vp]
+        L.song_mlp_loss_grad.argtypes = [vp, vp, vp, vp, i32, ctypes.c_float, vp, vp, vp]
+        L.song_mlp_adam_step.argtypes = [vp, ctypes.c_float, vp]
+        L.song_mlp_infer.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
+        L.song_mlp_features.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, i32, vp]
+        L.song_gemm_bf16.argtypes = [i32, i32, i32, vp, i32, i32, vp, i32, i32, vp, i32, ctypes.c_float, vp]
+        L._mlp_bound = True
+    return L
+
+
+def _mcheck(rc):
+    if rc != 0:
+        raise SongError(f"song mlp error {rc}: {_mlp_lib().song_mlp_last_error().decode(errors='replace')}")
+
+
+def _dp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def gemm_bf16(a, b, sam, sak, sbk, sbn, M, N, K, alpha=1.0):
+    """Test hook: C = alpha * A . B with bf16 tensors (int16 storage) and strides."""
+    import torch
+
+    c = torch.empty((M, N), dtype=torch.float32, device=a.device)
+    _mcheck(_mlp_lib().song_gemm_bf16(M, N, K, _dp(a), sam, sak, _dp(b), sbk, sbn, _dp(c), N, alpha, _stream()))
+    return c
+
+
+class MLP:
+    """ctypes binding of song_mlp (include/song_learned.h)."""
+
+    def __init__(self, in_dim, hidden=256, n_classes=4096, max_batch=1024, seed=0):
+        h = ctypes.c_void_p()
+        _mcheck(_mlp_lib().song_mlp_create(in_dim, hidden, n_classes, max_batch, seed, ctypes.byref(h)))
+        self._h = h
+        self.in_dim, self.hidden, self.n_classes, self.max_batch = in_dim, hidden, n_classes, max_batch
+        self.width = _mlp_lib().song_mlp_input_width(h)
+        n = ctypes.c_uint64()
+        _mcheck(_mlp_lib().song_mlp_param_count(h, ctypes.byref(n)))
+        self.n_params = n.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _mlp_lib().song_mlp_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        _mcheck(_mlp_lib().song_mlp_get_params(self._h, ctypes.c_void_p(p.ctypes.data)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _mcheck(_mlp_lib().song_mlp_set_params(self._h, ctypes.c_void_p(p.ctypes.data)))
+
+    def loss_grad(self, x, labels, stop_t, stop_weight=0.1, want_grads=False):
+        loss = np.zeros(2, dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32) if want_grads else None
+        _mcheck(_mlp_lib().song_mlp_loss_grad(self._h, _dp(x), _dp(labels), _dp(stop_t), x.shape[0], stop_weight,
+                                              ctypes.c_void_p(loss.ctypes.data),
+                                              ctypes.c_void_p(g.ctypes.data) if g is not None else None, _stream()))
+        return float(loss[0]), float(loss[1]), g
+
+    def train_step(self, x, labels, stop_t, lr, stop_weight=0.1):
+        _mcheck(_mlp_lib().song_mlp_loss_grad(self._h, _dp(x), _dp(labels), _dp(stop_t), x.shape[0], stop_weight,
+                                              None, None, _stream()))
+        _mcheck(_mlp_lib().song_mlp_adam_step(self._h, lr, _stream()))
+
+    def infer(self, x, logits=False):
+        import torch
+
+        B = x.shape[0]
+        top2 = torch.empty((B, 2), dtype=torch.int32, device=x.device)
+        conf = torch.empty(B, dtype=torch.float32, device=x.device)
+        stop = torch.empty(B, dtype=torch.float32, device=x.device)
+        lg = torch.empty((B, self.n_classes + 1), dtype=torch.float32, device=x.device) if logits else None
+        _mcheck(_mlp_lib().song_mlp_infer(self._h, _dp(x), B, _dp(top2), _dp(conf), _dp(stop), _dp(lg), _stream()))
+        return (top2, conf, stop, lg) if logits else (top2, conf, stop)
+
+
+def kmeans(x, k, iters=10, seed=0, chunk=1 << 16):
+    """Lloyd's k-means on a float32 cuda tensor; returns (centroids [k, d], counts [k])."""
+    import torch
+
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    c = x[torch.randperm(x.shape[0], generator=g)[:k].to(x.device)].clone()
+    for _ in range(iters):
+        cn = (c * c).sum(1)
+        assign = torch.empty(x.shape[0], dtype=torch.int64, device=x.device)
+        for s in range(0, x.shape[0], chunk):
+            xs = x[s:s + chunk]
+            assign[s:s + chunk] = torch.argmin(cn[None, :] - 2.0 * (xs @ c.T), dim=1)
+        sums = torch.zeros_like(c).index_add_(0, assign, x)
+        cnt = torch.bincount(assign, minlength=k).to(x.dtype)
+        nz = cnt > 0
+        c[nz] = sums[nz] / cnt[nz, None]
+    return c, cnt
+
+
+def nearest_rows(base, points, chunk=1 << 18):
+    """Index of the nearest base row (L2) to each point; base/points float32 cuda tensors."""
+    import torch
+
+    pn = (points * points).sum(1)
+    best_d = torch.full((points.shape[0],), float("inf"), device=points.device)
+    best_i = torch.zeros(points.shape[0], dtype=torch.int64, device=points.device)
+    for s in range(0, base.shape[0], chunk):
+        b = base[s:s + chunk]
+        d = pn[:, None] + (b * b).sum(1)[None, :] - 2.0 * (points @ b.T)
+        v, i = torch.min(d, dim=1)
+        upd = v < best_d
+        best_d = torch.where(upd, v, best_d)
+        best_i = torch.where(upd, i + s, best_i)
+    return best_i
+
+
+def nearest_centroids(points, c, k=2, chunk=1 << 15):
+    import torch
+
+    cn = (c * c).sum(1)
+    out = torch.empty((points.shape[0], k), dtype=torch.int64, device=points.device)
+    for s in range(0, points.shape[0], chunk):
+        d = cn[None, :] - 2.0 * (points[s:s + chunk] @ c.T)
+        out[s:s + chunk] = torch.topk(d, k, dim=1, largest=False).indices
+    return out
+
+
+class LearnedSearch:
+    """Entry/stop predictor over one index (see the module docstring)."""
+
+    def __init__(self, index, base, n_pivots=4096, hidden=256, seed=0):
+        import torch
+
+        self.index = index
+        self.dev = torch.device("cuda:0")
+        self.base_dev = torch.from_numpy(np.ascontiguousarray(base, dtype=np.float32)).to(self.dev)
+        self.n_pivots, self.hidden, self.seed = n_pivots, hidden, seed
+        self.dim = base.shape[1]
+        self.mlp = None
+        self.log = {}
+
+    # ---------------------------------------------------------------- features
+    def features(self, q_dev):
+        import torch
+
+        x = torch.empty((q_dev.shape[0], self.mlp.width), dtype=torch.int16, device=self.dev)
+        fs = np.ascontiguousarray(self.fscale, dtype=np.float32)
+        _mcheck(_mlp_lib().song_mlp_features(_dp(q_dev), q_dev.shape[0], self.dim, _dp(self.mean), _dp(self.inv_std),
+                                             _dp(self.v0), ctypes.c_void_p(fs.ctypes.data), _dp(x), self.mlp.width,
+                                             _stream()))
+        return x
+
+    def _feature_stats(self, q):
+        import torch
+
+        self.mean = q.mean(0).contiguous()
+        self.inv_std = (1.0 / q.std(0).clamp_min(1e-6)).contiguous()
+        self.v0 = self.base_dev[0].contiguous()
+        f = torch.stack([q.norm(dim=1), (q - self.v0).norm(dim=1), q.mean(1), q.std(1, unbiased=False),
+                         q.min(1).values, q.max(1).values, (q == 0).float().mean(1)], 1)
+        sd = f.std(0).clamp_min(1e-6).cpu().numpy()
+        self.fscale = np.concatenate([1.0 / sd, [1.0]]).astype(np.float32)
+
+    # ---------------------------------------------------------------- training
+    def fit(self, train_queries, pq_label=256, k=10, epochs=8, batch=1024, lr=2e-3, stop_weight=0.05,
+            sample=200_000, verbose=True):
+        import torch
+
+        t0 = time.time()
+        dev = self.dev
+        g = torch.Generator(device="cpu").manual_seed(self.seed)
+        n = self.base_dev.shape[0]
+        samp = self.base_dev[torch.randperm(n, generator=g)[:min(sample, n)].to(dev)]
+        self.centroids, _ = kmeans(samp, self.n_pivots, iters=10, seed=self.seed)
+        self.pivot_vertex = nearest_rows(self.base_dev, self.centroids).to(torch.int32)
+        t1 = time.time()
+
+        q = torch.from_numpy(np.ascontiguousarray(train_queries, dtype=np.float32)).to(dev)
+        nt = q.shape[0]
+        ids = torch.empty((nt, k), dtype=torch.int32, device=dev)
+        self.index.search_device(q, pq_label, k, ids)
+        top1 = self.base_dev[ids[:, 0].long()]
+        lab2 = nearest_centroids(top1, self.centroids, 2)
+        labels = lab2[:, 0].to(torch.int32).contiguous()
+        entry = self.pivot_vertex[lab2].contiguous()
+        stats = torch.empty((nt, 8), dtype=torch.int32, device=dev)
+        ids2 = torch.empty((nt, k), dtype=torch.int32, device=dev)
+        self.index.search_device(q, pq_label, k, ids2, stats=stats, entry=entry)
+        settle = stats[:, 4].float()
+        stop_t = torch.log2(1.0 + settle).contiguous()
+        t2 = time.time()
+
+        self._feature_stats(q)
+        self.mlp = MLP(self.dim + 8, self.hidden, self.n_pivots, max_batch=batch, seed=self.seed)
+        X = self.features(q)
+        nb = nt // batch
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(nt, generator=g).to(dev)
+            lr_ep = lr * 0.5 * (1 + math.cos(math.pi * ep / epochs))
+            for b in range(nb):
+                sel = perm[b * batch:(b + 1) * batch]
+                self.mlp.train_step(X[sel].contiguous(), labels[sel].contiguous(), stop_t[sel].contiguous(), lr_ep,
+                                    stop_weight)
+            xe, ms, _ = self.mlp.loss_grad(X[:batch].contiguous(), labels[:batch].contiguous(),
+                                           stop_t[:batch].contiguous(), stop_weight)
+            hist.append((xe, ms))
+            if verbose:
+                print(f"[learned] epoch {ep} xent={xe:.3f} stop_mse={ms:.3f}", flush=True)
+        torch.cuda.synchronize()
+        top2, conf, stop = self.mlp.infer(X[:min(nt, 20000)])
+        acc = (top2[:, 0] == labels[:top2.shape[0]]).float().mean().item()
+        self.log = {"kmeans_s": t1 - t0, "traces_s": t2 - t1, "train_s": time.time() - t2, "loss": hist,
+                    "train_top1_acc": acc, "mean_conf": conf.mean().item(),
+                    "mean_settle": settle.mean().item(), "pq_label": pq_label}
+        return self.log
+
+    # ---------------------------------------------------------------- search
+    def predict(self, q_dev):
+        x = self.features(q_dev)
+        top2, conf, stop = self.mlp.infer(x)
+        entry = self.pivot_vertex[top2.long()].contiguous()
+        return entry, conf, stop
+
+    def plan(self, q_dev, conf_threshold=0.9, step_scale=1.5, step_add=8):
+        """Entries, expansion caps and the confident / fallback split for a batch."""
+        import torch
+
+        entry, conf, stop = self.predict(q_dev)
+        steps = (step_scale * (torch.exp2(stop) - 1.0) + step_add).clamp(1, 1_000_000).to(torch.int32)
+        ok = conf >= conf_threshold
+        confident = torch.nonzero(ok).flatten().to(torch.int32)
+        fallback = torch.nonzero(~ok).flatten().to(torch.int32)
+        return entry, steps.contiguous(), confident, fallback
+
+    def search(self, q_dev, pq, k, ids, plan=None, pq_fallback=None, conf_threshold=0.9, step_scale=1.5, step_add=8,
+               storage=0, dists=None, stats=None):
+        """Learned search of a batch into ids (int32 cuda [nq, >=k]); returns the plan."""
+        if plan is None:
+            plan = self.plan(q_dev, conf_threshold, step_scale, step_add)
+        entry, steps, confident, fallback = plan
+        if confident.numel():
+            self.index.search_device(q_dev, pq, k, ids, dists=dists, stats=stats, entry=entry, steps=steps,
+                                     subset=confident, storage=storage, retry=0)
+        if fallback.numel():
+            self.index.search_device(q_dev, pq_fallback or pq, k, ids, dists=dists, stats=stats, subset=fallback,
+                                     storage=storage, retry=0)
+        return plan

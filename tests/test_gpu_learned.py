@@ -1,0 +1,140 @@
+"""Learned form on the GPU: MFMA GEMM, MLP gradients and fused inference against the
+float64 numpy oracle, training, and the ablation learned-vs-plain."""
+import numpy as np
+import pytest
+
+import mlp_ref
+import song_amd
+from song_amd import datasets, groundtruth, learned
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_tensor(a):
+    import torch
+
+    a32 = np.ascontiguousarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    code = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+    return torch.from_numpy(code.view(np.int16)).cuda(), mlp_ref.bf16(a32)
+
+
+@pytest.mark.parametrize("M,N,K", [(70, 45, 37), (64, 64, 16), (1, 130, 300), (257, 3, 5)])
+def test_gemm_matches_numpy(M, N, K):
+    rng = np.random.default_rng(M * 1000 + N)
+    a, ar = _bf16_tensor(rng.standard_normal((M, K)))
+    b, br = _bf16_tensor(rng.standard_normal((K, N)))
+    c = learned.gemm_bf16(a, b, K, 1, N, 1, M, N, K).cpu().numpy()
+    ref = ar @ br
+    assert np.allclose(c, ref, rtol=1e-4, atol=1e-4 * np.sqrt(K))
+    # transposed operands through strides: A given as [K, M], B as [N, K]
+    at, _ = _bf16_tensor(ar.T.copy())
+    bt, _ = _bf16_tensor(br.T.copy())
+    c2 = learned.gemm_bf16(at, bt, 1, M, 1, K, M, N, K).cpu().numpy()
+    assert np.allclose(c2, ref, rtol=1e-4, atol=1e-4 * np.sqrt(K))
+
+
+def _small_problem(seed=0, width_in=20, hidden=32, classes=50, B=64):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    m = learned.MLP(width_in, hidden, classes, max_batch=B, seed=seed)
+    p = m.get_params() * 1.5  # move away from the initial scale
+    m.set_params(p)
+    x, xr = _bf16_tensor(np.pad(rng.standard_normal((B, width_in)), ((0, 0), (0, m.width - width_in))))
+    labels = rng.integers(0, classes, B).astype(np.int32)
+    stop = rng.standard_normal(B).astype(np.float32)
+    params = mlp_ref.split(p, m.width, hidden, classes)
+    return m, x, xr, torch.from_numpy(labels).cuda(), torch.from_numpy(stop).cuda(), params, labels, stop
+
+
+def test_mlp_gradients_match_oracle():
+    m, x, xr, lab_d, stop_d, params, labels, stop = _small_problem()
+    w = 0.3
+    xe, ms, g = m.loss_grad(x, lab_d, stop_d, w, want_grads=True)
+    rxe, rms, rg = mlp_ref.loss_grad(params, xr, labels, stop, w, emulate_bf16=True)
+    assert abs(xe - rxe) < 1e-3 * max(1, rxe) and abs(ms - rms) < 1e-3 * max(1, rms)
+    got = mlp_ref.split(g, m.width, m.hidden, m.n_classes)
+    for t, (a, b) in enumerate(zip(got, rg)):
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert err < 2e-2, (t, err)
+
+
+def test_fused_inference_matches_oracle():
+    m, x, xr, _, _, params, _, _ = _small_problem(seed=1)
+    top2, conf, stop, logits = m.infer(x, logits=True)
+    _, _, Z = mlp_ref.forward(params, xr, emulate_bf16=True)
+    order, rconf, rstop = mlp_ref.top2_conf(Z)
+    assert np.allclose(logits.cpu().numpy(), Z, rtol=1e-3, atol=1e-3)
+    srt = np.sort(Z[:, :-1], axis=1)
+    clear = (srt[:, -1] - srt[:, -2] > 1e-2) & (srt[:, -2] - srt[:, -3] > 1e-2)
+    assert np.array_equal(top2.cpu().numpy()[clear], order[clear])
+    assert np.allclose(conf.cpu().numpy(), rconf, atol=2e-3)
+    assert np.allclose(stop.cpu().numpy(), rstop, rtol=1e-3, atol=1e-3)
+
+
+def test_training_learns_nearest_centre_classes():
+    import torch
+
+    rng = np.random.default_rng(7)
+    classes, dim, n = 64, 16, 8192
+    centres = rng.standard_normal((classes, dim)) * 3
+    lab = rng.integers(0, classes, n)
+    xs = centres[lab] + rng.standard_normal((n, dim)) * 0.3
+    m = learned.MLP(dim, 64, classes, max_batch=512, seed=3)
+    xpad = np.pad(xs, ((0, 0), (0, m.width - dim)))
+    x, _ = _bf16_tensor(xpad)
+    ld = torch.from_numpy(lab.astype(np.int32)).cuda()
+    st = torch.zeros(n, dtype=torch.float32, device="cuda")
+    first, _, _ = m.loss_grad(x[:512], ld[:512], st[:512])
+    for ep in range(6):
+        for b in range(0, n, 512):
+            m.train_step(x[b:b + 512], ld[b:b + 512], st[b:b + 512], 3e-3)
+    last, _, _ = m.loss_grad(x[:512], ld[:512], st[:512])
+    top2, conf, _ = m.infer(x)
+    acc = (top2[:, 0].cpu().numpy() == lab).mean()
+    print(f"[learned] xent {first:.3f} -> {last:.3f}, top1 accuracy {acc:.3f}", flush=True)
+    assert last < 0.5 * first and acc > 0.9
+
+
+@pytest.fixture(scope="module")
+def mix50k():
+    base, _ = datasets.mixture_l2(n_base=50000, n_query=1, dim=32, n_centers=100, seed=12)
+    train = datasets.mixture_l2_queries(20000, dim=32, n_centers=100, seed=12, stream=1)
+    test = datasets.mixture_l2_queries(1000, dim=32, n_centers=100, seed=12, stream=2)
+    graph = song_amd.build_graph(base, "l2", threads=8)
+    ix = song_amd.Index(base, graph, "l2")
+    return base, graph, ix, train, test
+
+
+def test_learned_entries_beat_vertex_zero(mix50k):
+    """Ablation: same index, queries and pq; learned start vertices vs vertex 0."""
+    import torch
+
+    base, graph, ix, train, test = mix50k
+    ls = learned.LearnedSearch(ix, base, n_pivots=256, hidden=64, seed=0)
+    log = ls.fit(train, pq_label=64, epochs=6, batch=512, verbose=False)
+    print(f"[learned] fit {log}", flush=True)
+    gt_i, gt_d = groundtruth.knn_fp64(base, test, 10, "l2")
+    q = torch.from_numpy(test).cuda()
+    k, pq = 10, 24
+    plain = torch.empty((q.shape[0], k), dtype=torch.int32, device="cuda")
+    st_p = torch.empty((q.shape[0], 8), dtype=torch.int32, device="cuda")
+    ix.search_device(q, pq, k, plain, stats=st_p)
+    r_plain = groundtruth.recall_at_k(plain.cpu().numpy().astype(np.int64), gt_d, 10, base, test)
+    lrn = torch.empty_like(plain)
+    st_l = torch.empty_like(st_p)
+    plan = ls.search(q, pq, k, lrn, conf_threshold=0.0, step_scale=1e6, stats=st_l)  # entries only, no cap
+    r_learn = groundtruth.recall_at_k(lrn.cpu().numpy().astype(np.int64), gt_d, 10, base, test)
+    capped = torch.empty_like(plain)
+    st_c = torch.empty_like(st_p)
+    ls.search(q, pq, k, capped, conf_threshold=0.0, step_scale=1.5, stats=st_c)
+    r_cap = groundtruth.recall_at_k(capped.cpu().numpy().astype(np.int64), gt_d, 10, base, test)
+    e_p, e_l, e_c = (s[:, 0].float().mean().item() for s in (st_p, st_l, st_c))
+    print(f"[learned] pq={pq} recall plain={r_plain:.4f} learned={r_learn:.4f} learned+stop={r_cap:.4f} "
+          f"expansions {e_p:.1f} / {e_l:.1f} / {e_c:.1f}", flush=True)
+    assert r_learn > r_plain
+    assert e_c <= e_l
+    # the confidence split: every query is served, fallback ones by the plain search
+    entry, steps, conf_ids, fb_ids = ls.plan(q, conf_threshold=0.9)
+    assert conf_ids.numel() + fb_ids.numel() == q.shape[0]

@@ -55,7 +55,7 @@ def test_distances_and_counters_match_oracle(case):
     ids = torch.empty((nq, pq), dtype=torch.int32, device=dev)
     ds = torch.empty((nq, pq), dtype=torch.float32, device=dev)
     fl = torch.empty(nq, dtype=torch.int32, device=dev)
-    st = torch.empty((nq, 4), dtype=torch.int32, device=dev)
+    st = torch.empty((nq, 8), dtype=torch.int32, device=dev)
     ix.search_device(q, pq, pq, ids, ds, fl, st, mode=song_amd.MODE_EXACT)
     torch.cuda.synchronize()
     g_ids = ids.cpu().numpy().astype(np.int64)
