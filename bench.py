@@ -1,15 +1,21 @@
 #!/usr/bin/env python
-"""Benchmark: QPS at recall@10 >= 0.95 on the 1M x 128 L2 workload, one GPU.
+"""Benchmark: QPS at recall@10 >= 0.95 on the 1M x 128 L2 workload, one GPU,
+learned + BF16 form against the plain form.
 
 Workload: 1,000,000 base vectors, dim 128, integer-valued mixture of 1,000
 Gaussians (seed 2), 10,000 queries from the same mixture, L2, degree-31 graph
-from the reference insertion build (byte-identical, built here by the parallel
+from the reference insertion build (byte-identical, built by the parallel
 builder), top-10.  A "step" is one search of the whole 10,000-query batch with
 queries, index and outputs resident in HBM.
 
-The pq sweep picks the smallest pq whose recall@10 (tie-aware, against a
-float64 brute force on the GPU) reaches the target; that pq is then timed.
-Prints one JSON line (see the keys below).
+Plain form: the bit-exact search from vertex 0; the smallest pq of the sweep
+whose recall@10 (tie-aware, against a float64 brute force on the GPU) reaches
+the target is timed.  Learned + BF16 form: BF16 rows, MLP-predicted start
+vertices and stop caps (trained here on 200,000 trace-labelled queries drawn
+from the same mixture, disjoint from the benchmark queries), confidence
+fallback to the plain full-budget search; the fastest (pq, stop scale) setting
+whose recall reaches the target is timed.  `value` is the learned + BF16 QPS.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -43,6 +49,18 @@ def cpu_model():
     return platform.processor()
 
 
+def timed(fn, reps, torch):
+    """median milliseconds of reps runs of fn, each between synchronizations"""
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(times))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,27 +68,31 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--pqs", default="64,96,128,160,192,224,256,320,384,448,512,640,768")
-    ap.add_argument("--pq", type=int, default=0, help="skip the sweep and time this pq")
+    ap.add_argument("--pqs", default="256,384,512,640,768,896,1024")
+    ap.add_argument("--pq", type=int, default=0, help="plain form: skip the sweep and time this pq")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="plain form only")
+    ap.add_argument("--train-queries", type=int, default=200_000)
+    ap.add_argument("--learned-pqs", default="64,96,128,192,256,384")
+    ap.add_argument("--step-scales", default="1.5,2.5,0")
     ap.add_argument("--no-recall", action="store_true",
                     help="with --pq: skip the ground truth and recall (profiling runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one device copy of this many MiB first (a known-size kernel that "
                          "calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
-    if args.no_recall and not args.pq:
-        ap.error("--no-recall needs --pq")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries shard across GPUs "
                                    "with no communication, see DESIGN.md)"}))
         sys.exit(2)
+    if args.no_recall and not args.pq:
+        ap.error("--no-recall needs --pq")
 
     import torch
 
     import song_amd
-    from song_amd import groundtruth, workloads
+    from song_amd import datasets, groundtruth, workloads
 
     dev = torch.device("cuda:0")
     t0 = time.time()
@@ -79,6 +101,7 @@ def main():
     ix = song_amd.Index(base, graph, metric)
     nq, dim = query.shape
     k = args.k
+    stream = torch.cuda.current_stream()
 
     if args.copy_mb:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device=dev)
@@ -88,87 +111,142 @@ def main():
         log(f"calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
         del src, dst
 
+    gt_d = None
     if not args.no_recall:
         t1 = time.time()
-        gt_i, gt_d = groundtruth.knn_fp64(base, query, k, metric)
+        _, gt_d = groundtruth.knn_fp64(base, query, k, metric)
         log(f"fp64 ground truth in {time.time() - t1:.1f}s")
 
+    def recall(ids_t):
+        if gt_d is None:
+            return float("nan")
+        return groundtruth.recall_at_k(ids_t.cpu().numpy().astype(np.int64), gt_d, k, base, query, metric)
+
     q = torch.from_numpy(query).to(dev)
-    stream = torch.cuda.current_stream()
-
-    def run(pq, ids, stats=None, retry=1):
-        ix.search_device(q, pq, k, ids, None, flags, stats, retry=retry)
-
     flags = torch.empty(nq, dtype=torch.int32, device=dev)
     stats = torch.empty((nq, 8), dtype=torch.int32, device=dev)
     ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    # integer-valued data and queries: FP32 accumulation is exact (same ids as the exact mode)
+    mode = song_amd.MODE_FP32 if ix.integer_valued and bool(np.all(np.rint(query) == query)) else song_amd.MODE_EXACT
 
+    def plain(pq, st=None, retry=0):
+        ix.search_device(q, pq, k, ids, None, flags, st, mode=mode, retry=retry)
+
+    # ---------------------------------------------------------------- plain form
     sweep = []
     chosen = None
     pqs = [args.pq] if args.pq else [int(x) for x in args.pqs.split(",")]
     for pq in pqs:
-        run(pq, ids, stats)
+        plain(pq, stats, retry=1)
         torch.cuda.synchronize()
-        r = (float("nan") if args.no_recall else
-             groundtruth.recall_at_k(ids.cpu().numpy().astype(np.int64), gt_d, k, base, query, metric))
+        r = recall(ids)
         st = stats.cpu().numpy()
-        times = []
-        for _ in range(3):
-            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            ev0.record(stream)
-            run(pq, ids, retry=0)
-            ev1.record(stream)
-            torch.cuda.synchronize()
-            times.append(ev0.elapsed_time(ev1))
-        ms = float(np.median(times))
+        ms = timed(lambda: plain(pq), 3, torch)
         sweep.append({"pq": pq, "recall@10": round(r, 4), "qps": round(nq / (ms / 1e3), 1),
                       "mean_expansions": round(float(st[:, 0].mean()), 1),
                       "mean_distances": round(float(st[:, 1].mean()), 1),
                       "prefetch_hit": round(float(st[:, 3].sum() / max(1, st[:, 0].sum())), 4)})
-        log(f"pq={pq} recall@{k}={r:.4f} median-of-3 {ms:.2f} ms ({nq / (ms / 1e3):.0f} q/s) "
-            f"exp={st[:, 0].mean():.1f} dist={st[:, 1].mean():.1f} pf={st[:, 3].sum() / max(1, st[:, 0].sum()):.3f}")
-        if chosen is None and r >= args.target_recall:
+        log(f"plain pq={pq} recall@{k}={r:.4f} {ms:.2f} ms ({nq / (ms / 1e3):.0f} q/s) "
+            f"exp={st[:, 0].mean():.1f} dist={st[:, 1].mean():.1f}")
+        if chosen is None and (r >= args.target_recall or args.no_recall):
             chosen = (pq, r, st)
             if not args.pq:
                 break
     if chosen is None:
         pq, r, st = pqs[-1], sweep[-1]["recall@10"], stats.cpu().numpy()
-        log(f"target recall not reached; timing pq={pq}")
+        log(f"plain form did not reach the target recall; timing pq={pq}")
     else:
         pq, r, st = chosen
+    retry = 1 if int((flags.cpu().numpy() & 3).sum()) > 0 else 0
 
-    # timed region: W untimed steps, then K steps between synchronizations
-    fl_host = flags.cpu().numpy()
-    overflow = int((fl_host & 3).sum() > 0)
-    retry = 1 if overflow else 0
+    # timed region of the plain form: W untimed steps, then K steps between synchronizations
     for _ in range(args.warmup):
-        run(pq, ids, retry=retry)
+        plain(pq, retry=retry)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t_start = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
-        run(pq, ids, retry=retry)
+        plain(pq, retry=retry)
     ev1.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    ms_total = ev0.elapsed_time(ev1)
-    ms_step = ms_total / args.steps
-    qps = nq * args.steps / (ms_total / 1e3)
+    ms_plain = ev0.elapsed_time(ev1) / args.steps
+    qps_plain = nq / (ms_plain / 1e3)
+    plain_info = {"pq": pq, "recall@10": round(r, 4), "qps": round(qps_plain, 1), "ms_per_step": round(ms_plain, 4),
+                  "mean_expansions": round(float(st[:, 0].mean()), 1),
+                  "mean_distances": round(float(st[:, 1].mean()), 1), "arithmetic": "fp32 (exact on integer data)"}
+    log(f"plain form: {qps_plain:.0f} q/s at pq={pq} recall={r:.4f}")
 
-    # algorithmic bytes of one launch: every distance reads one fp32 row, every expansion
-    # reads (1 + degree) u32 adjacency words, every query reads its own vector once
+    # algorithmic bytes of one plain launch: every distance reads one fp32 row, every
+    # expansion reads (1 + degree) u32 adjacency words, every query reads its own vector
     st64 = st.astype(np.int64)
-    alg_bytes = int(st64[:, 1].sum() * dim * 4 + st64[:, 2].sum() * 4 + nq * dim * 4)
-    achieved = alg_bytes / (ms_step / 1e3)
+    alg_plain = int(st64[:, 1].sum() * dim * 4 + st64[:, 2].sum() * 4 + nq * dim * 4)
 
+    # ---------------------------------------------------------------- learned + BF16 form
+    learned_info = None
+    if not args.no_learned and not args.no_recall:
+        from song_amd import learned
+
+        ix.add_storage(song_amd.STORE_BF16)
+        train = datasets.mixture_l2_queries(args.train_queries, seed=2, stream=1)
+        ls = learned.LearnedSearch(ix, base, n_pivots=4096, hidden=256, seed=0)
+        t2 = time.time()
+        fit = ls.fit(train, pq_label=256, k=k, epochs=8, batch=1024, lr=2e-3, verbose=False)
+        log(f"learned predictor trained in {time.time() - t2:.1f}s: top1 acc {fit['train_top1_acc']:.3f} "
+            f"mean conf {fit['mean_conf']:.3f} mean settle {fit['mean_settle']:.1f}")
+        lst = torch.empty((nq, 8), dtype=torch.int32, device=dev)
+        lids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+        lsweep = []
+        best = None
+        for lpq in [int(x) for x in args.learned_pqs.split(",")]:
+            for sc in [float(x) for x in args.step_scales.split(",")]:
+                scale = sc if sc > 0 else 1e6
+
+                def run(lpq=lpq, scale=scale):
+                    ls.search(q, lpq, k, lids, pq_fallback=pq, conf_threshold=0.9, step_scale=scale,
+                              storage=song_amd.STORE_BF16)
+
+                plan = ls.search(q, lpq, k, lids, pq_fallback=pq, conf_threshold=0.9, step_scale=scale,
+                                 storage=song_amd.STORE_BF16, stats=lst)
+                torch.cuda.synchronize()
+                rl = recall(lids)
+                ms = timed(run, 3, torch)
+                stl = lst.cpu().numpy()
+                ent = {"pq": lpq, "step_scale": sc, "recall@10": round(rl, 4), "qps": round(nq / (ms / 1e3), 1),
+                       "fallback_frac": round(plan[3].numel() / nq, 4),
+                       "mean_expansions": round(float(stl[:, 0].mean()), 1),
+                       "mean_distances": round(float(stl[:, 1].mean()), 1)}
+                lsweep.append(ent)
+                log(f"learned+bf16 pq={lpq} scale={sc} recall={rl:.4f} {ms:.2f} ms ({ent['qps']:.0f} q/s) "
+                    f"fallback={ent['fallback_frac']:.3f} exp={ent['mean_expansions']}")
+                if rl >= args.target_recall and (best is None or ent["qps"] > best[0]["qps"]):
+                    best = (ent, run, stl)
+        if best is not None:
+            ent, run, stl = best
+            for _ in range(args.warmup):
+                run()
+            torch.cuda.synchronize()
+            t_start = time.perf_counter()
+            for _ in range(args.steps):
+                run()
+            torch.cuda.synchronize()
+            ms_l = (time.perf_counter() - t_start) * 1e3 / args.steps
+            stl64 = stl.astype(np.int64)
+            alg_l = int(stl64[:, 1].sum() * (dim * 2 + 4) + stl64[:, 2].sum() * 4 + nq * dim * 4)
+            fit_out = {kk: vv for kk, vv in fit.items() if kk != "loss"}
+            fit_out["final_loss"] = fit["loss"][-1]
+            learned_info = dict(ent, qps=round(nq / (ms_l / 1e3), 1), ms_per_step=round(ms_l, 4),
+                                algorithmic_bytes_per_step=alg_l, training=fit_out)
+            log(f"learned+bf16 form: {learned_info['qps']:.0f} q/s ({learned_info['qps'] / qps_plain:.2f}x plain)")
+        learned_info = learned_info or {"error": "no learned setting reached the target recall"}
+        learned_info["sweep"] = lsweep
+
+    # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
 
         threads = min(16, os.cpu_count() or 1)
-        # size the sample from a short single-query probe to about --cpu-seconds of work
         tp = time.perf_counter()
         pyoracle.search_batch(base, graph, metric, query[:50], pq, threads=1)
         per_q = (time.perf_counter() - tp) / 50
@@ -176,6 +254,8 @@ def main():
         tc = time.perf_counter()
         cids, _ = pyoracle.search_batch(base, graph, metric, query[:sample], pq, threads=threads)
         tcpu = time.perf_counter() - tc
+        plain(pq, retry=1)
+        torch.cuda.synchronize()
         same = bool(np.array_equal(cids[:, :k].astype(np.int64), ids[:sample].cpu().numpy().astype(np.int64)))
         cpu = {"value": round(sample / tcpu, 1), "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": f"first {sample} of the {nq} queries at pq={pq} (oracle restatement of graph.h:169-216, "
@@ -183,9 +263,19 @@ def main():
                "cpu_model": cpu_model(), "ids_equal_gpu": same}
         log(f"cpu baseline: {cpu['value']} q/s on {threads} threads ({sample} queries)")
 
+    have_learned = bool(learned_info and "qps" in learned_info)
+    if have_learned:
+        value, ms_step, dtype = learned_info["qps"], learned_info["ms_per_step"], "bf16"
+        form = "learned + BF16 (MLP start vertices and stop caps, BF16 rows, exact FP32 re-rank, confidence fallback)"
+        alg, kern_ms = learned_info["algorithmic_bytes_per_step"], learned_info["ms_per_step"]
+    else:
+        value, ms_step, dtype = round(qps_plain, 1), ms_plain, "fp32"
+        form = "plain (bit-exact ids)"
+        alg, kern_ms = alg_plain, ms_plain
+    achieved_plain = alg_plain / (ms_plain / 1e3)
     out = {
-        "metric": "QPS @ recall@10>=0.95, 1M x128 L2, 1 GPU",
-        "value": round(qps, 1),
+        "metric": "QPS @ recall@10>=0.95, 1M x128 L2, 1 GPU: learned/BF16 form vs plain",
+        "value": value,
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -194,20 +284,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": dtype,
         "data": "synthetic: seeded Gaussian-mixture generator (integer-valued), graph = reference insertion build",
         "config": {"workload": "1,000,000 x 128 L2 mixture (1,000 centres, sigma 20, clipped/rounded), "
                                "10,000 queries, top-10",
-                   "form": "plain (exact ids; fp32 accumulation is exact on integer data)",
-                   "pq": pq, "k": k, "recall@10": round(r, 4), "target_recall": args.target_recall,
-                   "mean_expansions": round(float(st[:, 0].mean()), 1),
-                   "mean_distances": round(float(st[:, 1].mean()), 1),
-                   "wall_s_timed": round(wall, 4), "overflow_retry": bool(retry)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                   "form": form, "k": k, "target_recall": args.target_recall,
+                   "plain": plain_info,
+                   "learned_bf16": learned_info,
+                   "learned_vs_plain": round(value / qps_plain, 3) if have_learned else None,
+                   "plain_sweep": sweep},
+        "roofline": {"bound": "hbm", "achieved": round(achieved_plain / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK, 5), "traffic": None,
+                     "kernel": "search_kernel<L2, fp32, F32 rows> (plain form)",
+                     "algorithmic_bytes_per_launch": alg_plain,
+                     "learned_achieved_GBps": round(alg / (kern_ms / 1e3) / 1e9, 2)},
         "cpu_baseline": cpu,
-        "sweep": sweep,
     }
     print(json.dumps(out))
 

@@ -54,6 +54,14 @@ int song_mlp_infer(song_mlp *mlp, const uint16_t *d_x, int batch, int *d_top2, f
                    float *d_logits, void *stream);
 
 /*
+ * As song_mlp_infer, with the confidence taken as the softmax mass of the best pivot's
+ * neighbourhood d_nbr[best * n_nbr .. + n_nbr) (c x n_nbr pivot indices, each row
+ * starting with its own pivot) instead of the two best pivots.
+ */
+int song_mlp_infer_nbr(song_mlp *mlp, const uint16_t *d_x, int batch, const int *d_nbr, int n_nbr, int *d_top2,
+                       float *d_conf, float *d_stop, float *d_logits, void *stream);
+
+/*
  * Input features of the learned search, batch of queries (device, nq x dim float32):
  * columns [0, dim): (q - mean) * inv_std per coordinate; then 8 query-state features
  * scaled by fscale: |q|, distance to the default entry row v0, mean, std, min, max,

@@ -272,6 +272,8 @@ struct InferArgs {
     const float *b1, *b2, *b3;
     int *top2;  // batch x 2 pivot indices (best first)
     float *conf, *stop, *logits;
+    const int *nbr;  // C x n_nbr pivot neighbourhoods (row a starts with a) or null
+    int n_nbr;
 };
 
 // One dense layer on the LDS tile: accumulators of in[64 rows] . W[n]^T for the 64-column
@@ -432,7 +434,22 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
         const Soft t = run[tid];
         a.top2[2 * (r0 + tid)] = t.a1;
         a.top2[2 * (r0 + tid) + 1] = t.a2;
-        a.conf[r0 + tid] = (1.0f + (t.v2 == -INFINITY ? 0.0f : expf(t.v2 - t.v1))) / t.s;
+        float mass;
+        if (a.nbr) {
+            // softmax mass of the best pivot's neighbourhood: its logits recomputed from the
+            // last hidden layer (one thread per row)
+            mass = 0.0f;
+            for (int e = 0; e < a.n_nbr; ++e) {
+                const int p = a.nbr[(long long)t.a1 * a.n_nbr + e];
+                float z = a.b3[p];
+                const uint16_t *w = a.W3 + (long long)p * a.hidden;
+                for (int h = 0; h < a.hidden; ++h) z += bf2f(H2[tid * ldh + h]) * bf2f(w[h]);
+                mass += expf(fminf(z - t.v1, 0.0f));
+            }
+        } else {
+            mass = 1.0f + (t.v2 == -INFINITY ? 0.0f : expf(t.v2 - t.v1));
+        }
+        a.conf[r0 + tid] = fminf(mass / t.s, 1.0f);
         a.stop[r0 + tid] = stp[tid];
     }
 }
@@ -640,6 +657,12 @@ extern "C" int song_mlp_adam_step(song_mlp *m, float lr, void *stream) {
 
 extern "C" int song_mlp_infer(song_mlp *m, const uint16_t *d_x, int batch, int *d_top2, float *d_conf, float *d_stop,
                               float *d_logits, void *stream) {
+    return song_mlp_infer_nbr(m, d_x, batch, nullptr, 0, d_top2, d_conf, d_stop, d_logits, stream);
+}
+
+extern "C" int song_mlp_infer_nbr(song_mlp *m, const uint16_t *d_x, int batch, const int *d_nbr, int n_nbr, int *d_top2,
+                                  float *d_conf, float *d_stop, float *d_logits, void *stream) {
+    if (d_nbr && (n_nbr < 1 || n_nbr > 64)) return mfail(-1, "n_nbr=%d not in [1,64]", n_nbr);
     if (!m || !d_x || !d_top2 || !d_conf || !d_stop) return mfail(-1, "null argument");
     if (batch < 1) return 0;
     InferArgs a{};
@@ -658,6 +681,8 @@ extern "C" int song_mlp_infer(song_mlp *m, const uint16_t *d_x, int batch, int *
     a.conf = d_conf;
     a.stop = d_stop;
     a.logits = d_logits;
+    a.nbr = d_nbr;
+    a.n_nbr = n_nbr;
     const size_t lds = (size_t)IQ * (m->width + 8) * 2 + 2 * (size_t)IQ * (m->hidden + 8) * 2 + (5 * IQ) * sizeof(Soft) + IQ * 4;
     if (lds > 65536) MHIP(hipFuncSetAttribute((const void *)infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(infer_kernel, dim3((batch + IQ - 1) / IQ), dim3(256), lds, (hipStream_t)stream, a);

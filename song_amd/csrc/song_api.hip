@@ -8,7 +8,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/song_amd.h"
@@ -44,8 +46,14 @@ struct song_index {
     int integer_valued = 0;
     float vmin = 0.f, vmax = 0.f;
     size_t max_lds = 65536;
-    uint32_t *d_gvis = nullptr;  // visited spill tables, one per launch slot
-    size_t gvis_slots = 0;
+    // visited spill tables, one per launch slot, one set per stream: calls on one stream
+    // run in order; calls on different streams use different tables
+    struct Spill {
+        uint32_t *d = nullptr;
+        size_t slots = 0;
+    };
+    std::unordered_map<void *, Spill> spill;
+    std::mutex spill_mu;
     uint8_t *d_lp[3] = {nullptr, nullptr, nullptr};  // low-precision rows by SONG_STORE_*
     float *d_scale[3] = {nullptr, nullptr, nullptr};
     size_t lp_stride[3] = {0, 0, 0};
@@ -55,13 +63,21 @@ struct song_index {
 static constexpr int kGhlog = 13;
 static constexpr unsigned kChunk = 32768;
 
-static int ensure_gvis(song_index *ix, size_t slots) {
-    if (ix->gvis_slots >= slots) return SONG_OK;
-    if (ix->d_gvis) hipFree(ix->d_gvis);
-    ix->d_gvis = nullptr;
-    ix->gvis_slots = 0;
-    HIP_TRY(hipMalloc(&ix->d_gvis, slots * ((size_t)4 << kGhlog)));
-    ix->gvis_slots = slots;
+static int ensure_gvis(song_index *ix, void *stream, size_t slots, uint32_t **out) {
+    std::lock_guard<std::mutex> g(ix->spill_mu);
+    song_index::Spill &sp = ix->spill[stream];
+    if (sp.slots < slots) {
+        if (sp.d) {
+            // the previous buffer may still be read by work queued on this stream
+            HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+            HIP_TRY(hipFree(sp.d));
+        }
+        sp.d = nullptr;
+        sp.slots = 0;
+        HIP_TRY(hipMalloc(&sp.d, slots * ((size_t)4 << kGhlog)));
+        sp.slots = slots;
+    }
+    *out = sp.d;
     return SONG_OK;
 }
 
@@ -282,7 +298,8 @@ extern "C" int song_index_destroy(song_index *ix) {
     if (ix->d_data) hipFree(ix->d_data);
     if (ix->d_graph) hipFree(ix->d_graph);
     if (ix->d_norms) hipFree(ix->d_norms);
-    if (ix->d_gvis) hipFree(ix->d_gvis);
+    for (auto &kv : ix->spill)
+        if (kv.second.d) hipFree(kv.second.d);
     delete ix;
     return SONG_OK;
 }
@@ -333,12 +350,13 @@ static SearchFn search_fn(int metric, const Plan &p) {
     }
 }
 
-static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned count, hipStream_t st) {
+static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned count, hipStream_t st,
+                  uint32_t *gvis) {
     SearchFn fn = search_fn(ix->metric, p);
     sp.hlog = p.hlog;
     sp.slack = p.slack;
     sp.qcap = p.qcap;
-    sp.gvis = ix->d_gvis;
+    sp.gvis = gvis;
     sp.ghlog = kGhlog;
     sp.qld = p.qld;
     sp.data_lp = ix->d_lp[p.store];
@@ -418,7 +436,8 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
         exact = !fp32_exact(ix, qint, qmin, qmax);
     }
 
-    int rcg = ensure_gvis(ix, (size_t)std::min<uint64_t>(nq, kChunk));
+    uint32_t *gvis = nullptr;
+    int rcg = ensure_gvis(ix, stream, (size_t)std::min<uint64_t>(nrun ? nrun : 1, kChunk), &gvis);
     if (rcg != SONG_OK) return rcg;
     int hlog = opts.hash_log2 > 0 ? opts.hash_log2 : default_hlog(pq);
     hlog = std::max(9, std::min(hlog, 16));
@@ -461,7 +480,7 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
         if (tmp_flags) hipFree(tmp_flags);
         return SONG_OK;
     }
-    rc = launch(ix, plan, sp, (unsigned)nrun, st);
+    rc = launch(ix, plan, sp, (unsigned)nrun, st, gvis);
     if (rc == SONG_OK && opts.retry) {
         // re-run queries whose LDS tables overflowed, with 4x larger tables, until none remain
         std::vector<uint32_t> hf(nq);
@@ -509,7 +528,7 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
             }
             SearchParams sp2 = sp;
             sp2.qidx = d_idx;
-            rc = launch(ix, plan, sp2, (unsigned)redo.size(), st);
+            rc = launch(ix, plan, sp2, (unsigned)redo.size(), st, gvis);
             if (rc != SONG_OK) break;
         }
         if (d_idx) hipFree(d_idx);

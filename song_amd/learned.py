@@ -10,15 +10,16 @@ Pipeline (all heavy work on the GPU):
   traces   the plain search (from vertex 0) of the training queries; the label
            of a query is the centroid nearest to its best result, and its stop
            target is log2(1 + settle step) of a search started at the label's
-           two nearest pivots (settle step = the last expansion that changed
-           the top-k, reported by the search kernel).
+           neighbourhood (settle step = the last expansion that changed the
+           top-k, reported by the search kernel).
   network  song_learned.h: dim + 8 features -> hidden -> hidden -> n_pivots + 1,
            BF16 MFMA forward/backward kernels, Adam.
-  search   confident queries (softmax mass of the two best pivots >= threshold)
-           start at those two pivot vertices with budget pq and an expansion
-           cap from the stop head; the rest fall back to the plain full-budget
-           search from vertex 0.  Both groups run in one call each through
-           song_search_device_ex with a query subset.
+  search   confident queries (softmax mass of the best pivot's neighbourhood --
+           it and its 3 nearest pivots -- at least the threshold) start at the
+           neighbourhood's pivot vertices with budget pq and an expansion cap
+           from the stop head; the rest fall back to the plain full-budget
+           search from vertex 0.  The two groups run as two calls of
+           song_search_device_ex with query subsets, on two streams.
 """
 from __future__ import annotations
 
@@ -45,6 +46,7 @@ def _mlp_lib():
         L.song_mlp_loss_grad.argtypes = [vp, vp, vp, vp, i32, ctypes.c_float, vp, vp, vp]
         L.song_mlp_adam_step.argtypes = [vp, ctypes.c_float, vp]
         L.song_mlp_infer.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
+        L.song_mlp_infer_nbr.argtypes = [vp, vp, i32, vp, i32, vp, vp, vp, vp, vp]
         L.song_mlp_features.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, i32, vp]
         L.song_gemm_bf16.argtypes = [i32, i32, i32, vp, i32, i32, vp, i32, i32, vp, i32, ctypes.c_float, vp]
         L._mlp_bound = True
@@ -118,7 +120,9 @@ class MLP:
                                               None, None, _stream()))
         _mcheck(_mlp_lib().song_mlp_adam_step(self._h, lr, _stream()))
 
-    def infer(self, x, logits=False):
+    def infer(self, x, logits=False, nbr=None):
+        """Fused inference: (top2 pivots, confidence, stop[, logits]).  With nbr (int32
+        [n_classes, E]) the confidence is the mass of the best pivot's neighbourhood."""
         import torch
 
         B = x.shape[0]
@@ -126,7 +130,8 @@ class MLP:
         conf = torch.empty(B, dtype=torch.float32, device=x.device)
         stop = torch.empty(B, dtype=torch.float32, device=x.device)
         lg = torch.empty((B, self.n_classes + 1), dtype=torch.float32, device=x.device) if logits else None
-        _mcheck(_mlp_lib().song_mlp_infer(self._h, _dp(x), B, _dp(top2), _dp(conf), _dp(stop), _dp(lg), _stream()))
+        _mcheck(_mlp_lib().song_mlp_infer_nbr(self._h, _dp(x), B, _dp(nbr), nbr.shape[1] if nbr is not None else 0,
+                                              _dp(top2), _dp(conf), _dp(stop), _dp(lg), _stream()))
         return (top2, conf, stop, lg) if logits else (top2, conf, stop)
 
 
@@ -180,13 +185,13 @@ def nearest_centroids(points, c, k=2, chunk=1 << 15):
 class LearnedSearch:
     """Entry/stop predictor over one index (see the module docstring)."""
 
-    def __init__(self, index, base, n_pivots=4096, hidden=256, seed=0):
+    def __init__(self, index, base, n_pivots=4096, hidden=256, seed=0, n_nbr=4):
         import torch
 
         self.index = index
         self.dev = torch.device("cuda:0")
         self.base_dev = torch.from_numpy(np.ascontiguousarray(base, dtype=np.float32)).to(self.dev)
-        self.n_pivots, self.hidden, self.seed = n_pivots, hidden, seed
+        self.n_pivots, self.hidden, self.seed, self.n_nbr = n_pivots, hidden, seed, n_nbr
         self.dim = base.shape[1]
         self.mlp = None
         self.log = {}
@@ -225,6 +230,8 @@ class LearnedSearch:
         samp = self.base_dev[torch.randperm(n, generator=g)[:min(sample, n)].to(dev)]
         self.centroids, _ = kmeans(samp, self.n_pivots, iters=10, seed=self.seed)
         self.pivot_vertex = nearest_rows(self.base_dev, self.centroids).to(torch.int32)
+        # neighbourhood of each pivot: itself and its n_nbr - 1 nearest centroids
+        self.nbr = nearest_centroids(self.centroids, self.centroids, self.n_nbr).to(torch.int32).contiguous()
         t1 = time.time()
 
         q = torch.from_numpy(np.ascontiguousarray(train_queries, dtype=np.float32)).to(dev)
@@ -232,9 +239,8 @@ class LearnedSearch:
         ids = torch.empty((nt, k), dtype=torch.int32, device=dev)
         self.index.search_device(q, pq_label, k, ids)
         top1 = self.base_dev[ids[:, 0].long()]
-        lab2 = nearest_centroids(top1, self.centroids, 2)
-        labels = lab2[:, 0].to(torch.int32).contiguous()
-        entry = self.pivot_vertex[lab2].contiguous()
+        labels = nearest_centroids(top1, self.centroids, 1)[:, 0].to(torch.int32).contiguous()
+        entry = self.pivot_vertex[self.nbr[labels.long()].long()].contiguous()
         stats = torch.empty((nt, 8), dtype=torch.int32, device=dev)
         ids2 = torch.empty((nt, k), dtype=torch.int32, device=dev)
         self.index.search_device(q, pq_label, k, ids2, stats=stats, entry=entry)
@@ -260,7 +266,7 @@ class LearnedSearch:
             if verbose:
                 print(f"[learned] epoch {ep} xent={xe:.3f} stop_mse={ms:.3f}", flush=True)
         torch.cuda.synchronize()
-        top2, conf, stop = self.mlp.infer(X[:min(nt, 20000)])
+        top2, conf, stop = self.mlp.infer(X[:min(nt, 20000)], nbr=self.nbr)
         acc = (top2[:, 0] == labels[:top2.shape[0]]).float().mean().item()
         self.log = {"kmeans_s": t1 - t0, "traces_s": t2 - t1, "train_s": time.time() - t2, "loss": hist,
                     "train_top1_acc": acc, "mean_conf": conf.mean().item(),
@@ -269,9 +275,10 @@ class LearnedSearch:
 
     # ---------------------------------------------------------------- search
     def predict(self, q_dev):
+        """(start vertices [nq, n_nbr], confidence, stop output) of a batch."""
         x = self.features(q_dev)
-        top2, conf, stop = self.mlp.infer(x)
-        entry = self.pivot_vertex[top2.long()].contiguous()
+        top2, conf, stop = self.mlp.infer(x, nbr=self.nbr)
+        entry = self.pivot_vertex[self.nbr[top2[:, 0].long()].long()].contiguous()
         return entry, conf, stop
 
     def plan(self, q_dev, conf_threshold=0.9, step_scale=1.5, step_add=8):
@@ -290,11 +297,27 @@ class LearnedSearch:
         """Learned search of a batch into ids (int32 cuda [nq, >=k]); returns the plan."""
         if plan is None:
             plan = self.plan(q_dev, conf_threshold, step_scale, step_add)
+        import torch
+
         entry, steps, confident, fallback = plan
+        main = torch.cuda.current_stream()
+        if fallback.numel():
+            # the fallback group runs beside the confident one on a second stream
+            side = self._side_stream()
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                self.index.search_device(q_dev, pq_fallback or pq, k, ids, dists=dists, stats=stats,
+                                         subset=fallback, storage=storage, retry=0)
         if confident.numel():
             self.index.search_device(q_dev, pq, k, ids, dists=dists, stats=stats, entry=entry, steps=steps,
                                      subset=confident, storage=storage, retry=0)
         if fallback.numel():
-            self.index.search_device(q_dev, pq_fallback or pq, k, ids, dists=dists, stats=stats, subset=fallback,
-                                     storage=storage, retry=0)
+            main.wait_stream(self._side_stream())
         return plan
+
+    def _side_stream(self):
+        import torch
+
+        if getattr(self, "_side", None) is None:
+            self._side = torch.cuda.Stream()
+        return self._side

@@ -73,6 +73,22 @@ def test_fused_inference_matches_oracle():
     assert np.allclose(stop.cpu().numpy(), rstop, rtol=1e-3, atol=1e-3)
 
 
+def test_fused_inference_neighbourhood_confidence():
+    import torch
+
+    m, x, xr, _, _, params, _, _ = _small_problem(seed=2)
+    C, E = m.n_classes, 3
+    nbr = np.stack([(np.arange(C) + j * 7) % C for j in range(E)], 1).astype(np.int32)
+    top2, conf, _ = m.infer(x, nbr=torch.from_numpy(nbr).cuda())
+    _, _, Z = mlp_ref.forward(params, xr, emulate_bf16=True)
+    logits = Z[:, :-1]
+    p = np.exp(logits - logits.max(1, keepdims=True))
+    p /= p.sum(1, keepdims=True)
+    best = top2[:, 0].cpu().numpy()
+    ref = np.array([p[i, nbr[best[i]]].sum() for i in range(len(best))])
+    assert np.allclose(conf.cpu().numpy(), np.minimum(ref, 1.0), atol=2e-3)
+
+
 def test_training_learns_nearest_centre_classes():
     import torch
 
