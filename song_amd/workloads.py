@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import hashlib
 import os
+import sys
 import time
 
 import numpy as np
@@ -47,7 +48,8 @@ def graph_for(base: np.ndarray, metric: str, threads: int = 0, verbose: bool = F
     g, st = build_graph(base, metric, threads=threads, return_stats=True)
     st["cached"] = False
     if verbose:
-        print(f"[graph] built {n}x{base.shape[1]} {metric} in {time.time() - t:.1f}s ({st})", flush=True)
+        print(f"[graph] built {n}x{base.shape[1]} {metric} in {time.time() - t:.1f}s ({st})", file=sys.stderr,
+              flush=True)
     os.makedirs(CACHE_DIR, exist_ok=True)
     tmp = path + f".tmp{os.getpid()}"
     g.tofile(tmp)
