@@ -1,0 +1,106 @@
+// `song` command line, drop-in for the reference binaries (main.cu:158-205, main.cc:49-105):
+//   song <build/test> <build_data> <query_data> <search_top_k> <row> <dim> <return_top_k> <l2/ip/cos>
+// build: parse the one-based LIBSVM rows, build the graph (byte-identical to the reference
+//        insertion build), write bfsg.data and bfsg.graph to the working directory.
+// test:  load bfsg.data / bfsg.graph, search every query of the LIBSVM file in one GPU
+//        batch, print the first <return_top_k> ids of each result, "%zu " each, one line
+//        per query (main.cu:133-138).
+// The search budget pq is SONG_PQ when fill_parameters.sh compiled one in (the reference
+// GPU binary's TOPK), otherwise <search_top_k> (the reference CPU binary's k).
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <string>
+
+#include "kernelgraph.hpp"
+#if __has_include("song_params.h")
+#include "song_params.h"
+#endif
+
+using namespace song_cli;
+
+static void usage(char **argv) {
+    printf("Usage: %s <build/test> <build_data> <query_data> <search_top_k> <row> <dim> <return_top_k> <l2/ip/cos>\n",
+           argv[0]);
+}
+
+template <int M>
+static int run(const std::string &mode, char **argv, size_t row, int dim, int pq, int display) {
+    KernelFixedDegreeGraph<M> g(row, dim, pq);
+    if (mode == "build") {
+        uint64_t got = 0;
+        check_host(song_libsvm_parse(argv[2], 1, row, dim, g.data(), &got), "parse");
+        g.set_rows(got);
+        fprintf(stderr, "Writing the graph and data...");
+        g.dump_data();
+        fprintf(stderr, "...");
+        g.dump();
+        fprintf(stderr, "done\n");
+        return 0;
+    }
+    fprintf(stderr, "Loading the graph and data...");
+    g.load_data();
+    fprintf(stderr, "...");
+    g.load();
+    fprintf(stderr, "done\n");
+    // the query file: count lines, parse into dense rows, search as one batch
+    FILE *fp = fopen(argv[3], "r");
+    if (!fp) {
+        fprintf(stderr, "File not found at (%s)\n", argv[3]);
+        return 1;
+    }
+    size_t lines = 0;
+    for (int c; (c = fgetc(fp)) != EOF;)
+        if (c == '\n') ++lines;
+    fclose(fp);
+    std::vector<float> dense(std::max<size_t>(lines, 1) * (size_t)dim);
+    uint64_t nq = 0;
+    check_host(song_libsvm_parse(argv[3], 1, std::max<size_t>(lines, 1), dim, dense.data(), &nq), "parse queries");
+    std::vector<std::vector<std::pair<int, value_t>>> queries(nq);
+    for (uint64_t i = 0; i < nq; ++i)
+        for (int j = 0; j < dim; ++j)
+            if (dense[i * dim + j] != 0.0f) queries[i].push_back({j, dense[i * dim + j]});
+    std::vector<std::vector<idx_t>> results;
+    g.search_top_k_batch(queries, pq, results);
+    for (const auto &r : results) {
+        for (size_t j = 0; j < r.size() && (int)j < display; ++j) printf("%zu ", r[j]);
+        printf("\n");
+    }
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    if (argc != 9) {
+        usage(argv);
+        return 1;
+    }
+    const size_t row = atoll(argv[5]);
+    const int dim = atoi(argv[6]);
+    const int display = atoi(argv[7]);
+    const std::string metric = argv[8], mode = argv[1];
+    int pq = atoi(argv[4]);
+#ifdef SONG_PQ
+    pq = SONG_PQ;
+#endif
+#ifdef SONG_DIM
+    if (dim != SONG_DIM) fprintf(stderr, "warning: dim %d differs from the compiled %d\n", dim, SONG_DIM);
+#endif
+    if ((mode != "build" && mode != "test") || row == 0 || dim <= 0) {
+        usage(argv);
+        return 1;
+    }
+    if (mode == "test" && pq <= 0) {
+        fprintf(stderr, "pq (search_top_k) must be positive\n");
+        return 1;
+    }
+    try {
+        if (metric == "l2") return run<0>(mode, argv, row, dim, pq, display);
+        if (metric == "ip") return run<1>(mode, argv, row, dim, pq, display);
+        if (metric == "cos") return run<2>(mode, argv, row, dim, pq, display);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    usage(argv);
+    return 1;
+}
