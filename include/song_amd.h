@@ -150,6 +150,81 @@ int song_search_device_ex(song_index *index, const float *d_queries, uint64_t nq
                           uint32_t *d_stats, const song_search_opts *opts, void *stream);
 
 /*
+ * Batching form.  No counterpart in the reference, whose launch is one block per query
+ * with the compiled TOPK for all (warp_astar_accelerator.h:386); it replaces that launch
+ * for batches with uneven k and pq per query.
+ *
+ * A batch object owns every buffer its launches use, so song_batch_run only enqueues
+ * kernels (no allocation, no memset, no host synchronization) and can be captured in a
+ * hipGraph; song_batch_run_graph captures it once per argument set and replays it.
+ * One run: a bucket kernel sorts the queries into tiers by their pq, one persistent
+ * kernel per tier searches its list (waves take queries from a device work counter,
+ * LDS sized for the tier), and one persistent kernel re-runs queries that overflowed
+ * with the retry configuration.  Each query's result equals song_search_device's at the
+ * same (k, pq), entries, step cap, mode and storage.
+ */
+#define SONG_MAX_TIERS 8
+
+enum { SONG_FLAG_BAD_PARAMS = 8u }; /* batch: pq outside every tier (query not searched) */
+
+typedef struct {
+    int pq_max;          /* tier of the queries with pq in (previous pq_max, pq_max] */
+    int hash_log2;       /* visited filter: 2^hash_log2 LDS slots per wave (8..16), spilling to a
+                            global table; 0 = sized from pq_max */
+    int slack;           /* heap depth: queue entries kept beyond pq for distance ties (>= 0) */
+    int shape;           /* lanes per candidate row: 0..4 = 4x2, 4x8, 4x16, 16x16, 64x16
+                            (lanes x 16-byte units); -1 = by row width */
+    int waves_per_group; /* queries in flight per workgroup, one per wave: 1, 2 or 4 */
+    int groups_per_cu;   /* persistent grid = groups_per_cu x compute units; 0 = all that fit */
+} song_tier;
+
+typedef struct {
+    int mode;      /* SONG_MODE_EXACT or SONG_MODE_FP32 (song_resolve_mode picks like AUTO) */
+    int storage;   /* SONG_STORE_* */
+    int rerank;    /* low-precision storage: exact FP32 re-rank of the pq list */
+    int max_steps; /* expansion cap of every query; 0 = 1,000,000 */
+    int spill_log2; /* global visited spill table per wave: 2^spill_log2 u32 (10..18) */
+    int n_tiers;   /* 1..SONG_MAX_TIERS, ascending pq_max */
+    song_tier tier[SONG_MAX_TIERS];
+    song_tier retry; /* re-run of overflowed queries (pq_max ignored: the largest tier's) */
+} song_batch_config;
+
+typedef struct song_batch song_batch;
+
+/* One tier up to pq_max with default knobs; retry with 4x the filter and heap depth. */
+void song_batch_config_default(song_batch_config *cfg, int pq_max);
+
+/* The mode SONG_MODE_AUTO would pick for these device queries (synchronizes stream). */
+int song_resolve_mode(const song_index *index, const float *d_queries, uint64_t nq, int *mode, void *stream);
+
+/* Allocate a batch for at most nq_max queries per run. */
+int song_batch_create(song_index *index, const song_batch_config *cfg, uint64_t nq_max, song_batch **out);
+int song_batch_destroy(song_batch *batch);
+
+/*
+ * Enqueue one batched search on `stream` (a hipStream_t; NULL = default).  Arguments as
+ * song_search_device_ex (qp nullable; qp->subset unsupported); pq / k apply to queries
+ * without per-query values.  Per-query entry lists may be padded with 0xFFFFFFFF.  A
+ * query whose pq fits no tier gets SONG_FLAG_BAD_PARAMS and 0xFFFFFFFF ids; one that
+ * still overflows after the retry keeps its overflow flag.  d_flags is required.
+ * Runs of one batch object must not overlap (they share its buffers).
+ */
+int song_batch_run(song_batch *batch, const float *d_queries, uint64_t nq, int pq, int k, const song_query_params *qp,
+                   uint32_t *d_ids, float *d_dist, int k_ld, uint32_t *d_flags, uint32_t *d_stats, void *stream);
+
+/* song_batch_run through a hipGraph: captured on the first call with these arguments
+ * (stream must not be NULL), replayed by later calls with the same arguments. */
+int song_batch_run_graph(song_batch *batch, const float *d_queries, uint64_t nq, int pq, int k,
+                         const song_query_params *qp, uint32_t *d_ids, float *d_dist, int k_ld, uint32_t *d_flags,
+                         uint32_t *d_stats, void *stream);
+
+/* Queries per tier and re-run count of the last run (n_tiers + 1 values; synchronizes). */
+int song_batch_counts(song_batch *batch, uint32_t *out, void *stream);
+
+/* Persistent grid chosen for tier t (t = n_tiers: the retry), in workgroups. */
+int song_batch_grid(const song_batch *batch, int t, int *groups, int *waves_per_group, int *lds_per_wave);
+
+/*
  * One-shot replacement of WarpAStarAccelerator::astar_multi_start_search_batch
  * (warp_astar_accelerator.h:347): dense queries (nq x dim), pq = the compiled
  * TOPK of the reference, results nq x pq uint64 ascending by (dist, id).

@@ -39,6 +39,20 @@ class SearchOpts(ctypes.Structure):
                 ("rerank", ctypes.c_int)]
 
 
+class Tier(ctypes.Structure):
+    _fields_ = [("pq_max", ctypes.c_int), ("hash_log2", ctypes.c_int), ("slack", ctypes.c_int), ("shape", ctypes.c_int),
+                ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int)]
+
+
+MAX_TIERS = 8
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("storage", ctypes.c_int), ("rerank", ctypes.c_int),
+                ("max_steps", ctypes.c_int), ("spill_log2", ctypes.c_int), ("n_tiers", ctypes.c_int),
+                ("tier", Tier * MAX_TIERS), ("retry", Tier)]
+
+
 def lib():
     """Load the HIP library (raises SongError if it is not built)."""
     global _lib
@@ -67,6 +81,15 @@ def lib():
                                             vp, ctypes.POINTER(SearchOpts), vp]
         L.song_index_add_storage.argtypes = [vp, i32]
         L.song_index_get_storage.argtypes = [vp, i32, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.song_batch_config_default.argtypes = [ctypes.POINTER(BatchConfig), i32]
+        L.song_resolve_mode.argtypes = [vp, vp, u64, ctypes.POINTER(i32), vp]
+        L.song_batch_create.argtypes = [vp, ctypes.POINTER(BatchConfig), u64, ctypes.POINTER(vp)]
+        L.song_batch_destroy.argtypes = [vp]
+        run_args = [vp, vp, u64, i32, i32, ctypes.POINTER(QueryParams), vp, vp, i32, vp, vp, vp]
+        L.song_batch_run.argtypes = run_args
+        L.song_batch_run_graph.argtypes = run_args
+        L.song_batch_counts.argtypes = [vp, vp, vp]
+        L.song_batch_grid.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -75,7 +98,10 @@ EXPORTED_SYMBOLS = (
     "song_search_opts_default", "song_index_create", "song_index_destroy", "song_index_integer_valued",
     "song_search", "song_search_device", "song_astar_multi_start_search_batch", "song_last_error",
     "song_index_add_storage", "song_index_get_storage", "song_search_device_ex",
+    "song_batch_config_default", "song_resolve_mode", "song_batch_create", "song_batch_destroy", "song_batch_run",
+    "song_batch_run_graph", "song_batch_counts", "song_batch_grid",
 )
+FLAG_BAD_PARAMS = 8
 
 
 def _check(rc):
@@ -180,6 +206,111 @@ class Index:
         _check(lib().song_search_device_ex(self._h, dp(queries), nq, pq, k, ctypes.byref(qp), dp(ids), dp(dists),
                                            ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
                                            ctypes.c_void_p(stream)))
+
+
+def _query_params(pq_arr=None, k_arr=None, entry=None, steps=None, subset=None):
+    qp = QueryParams()
+    qp.pq = pq_arr.data_ptr() if pq_arr is not None else None
+    qp.k = k_arr.data_ptr() if k_arr is not None else None
+    qp.entry = entry.data_ptr() if entry is not None else None
+    qp.n_entry = 1 if entry is None else (entry.shape[1] if entry.dim() == 2 else 1)
+    qp.steps = steps.data_ptr() if steps is not None else None
+    qp.subset = subset.data_ptr() if subset is not None else None
+    qp.n_subset = subset.numel() if subset is not None else 0
+    return qp
+
+
+def resolve_mode(index, queries, stream=None):
+    """MODE_EXACT or MODE_FP32 as MODE_AUTO would pick for these device queries."""
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    m = ctypes.c_int()
+    _check(lib().song_resolve_mode(index._h, ctypes.c_void_p(queries.data_ptr()), queries.shape[0], ctypes.byref(m),
+                                   ctypes.c_void_p(stream)))
+    return m.value
+
+
+TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu")
+
+
+def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0):
+    """One tier of a batch configuration (see song_tier in include/song_amd.h)."""
+    return dict(pq_max=pq_max, hash_log2=hash_log2, slack=slack, shape=shape, waves_per_group=waves_per_group,
+                groups_per_cu=groups_per_cu)
+
+
+class Batch:
+    """Batching form over an Index: persistent per-tier kernels for queries with uneven k / pq.
+
+    tiers: list of tier(...) dicts with ascending pq_max (a query goes to the first tier
+    whose pq_max >= its pq); retry: a tier(...) dict for the re-run of overflowed queries
+    (None: 4x the largest tier's filter and heap depth)."""
+
+    def __init__(self, index, tiers, nq_max, mode=MODE_EXACT, storage=STORE_F32, rerank=1, max_steps=0,
+                 spill_log2=13, retry=None):
+        if not 1 <= len(tiers) <= MAX_TIERS:
+            raise SongError(f"{len(tiers)} tiers (1..{MAX_TIERS})")
+        cfg = BatchConfig()
+        lib().song_batch_config_default(ctypes.byref(cfg), int(tiers[-1]["pq_max"]))
+        cfg.mode, cfg.storage, cfg.rerank, cfg.max_steps, cfg.spill_log2 = mode, storage, rerank, max_steps, spill_log2
+        cfg.n_tiers = len(tiers)
+        for i, t in enumerate(tiers):
+            for key in TIER_KEYS:
+                setattr(cfg.tier[i], key, int(t.get(key, getattr(cfg.tier[0], key) if key != "pq_max" else 0)))
+        if retry is not None:
+            for key in TIER_KEYS:
+                if key in retry:
+                    setattr(cfg.retry, key, int(retry[key]))
+        self.index, self.tiers, self.nq_max, self.cfg = index, [dict(t) for t in tiers], nq_max, cfg
+        h = ctypes.c_void_p()
+        _check(lib().song_batch_create(index._h, ctypes.byref(cfg), nq_max, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().song_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, queries, ids, flags, pq=None, k=10, pq_arr=None, k_arr=None, dists=None, stats=None, entry=None,
+            steps=None, graph=False, stream=None):
+        """Enqueue one run (torch cuda tensors): queries float32 [nq, dim]; ids int32 [nq, k_ld];
+        flags int32 [nq]; pq_arr / k_arr / steps int32 [nq]; entry int32 [nq, E] (-1 pads);
+        graph=True captures the run once per argument set and replays it."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        if pq is None:
+            pq = self.tiers[-1]["pq_max"]
+        dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        qp = _query_params(pq_arr, k_arr, entry, steps)
+        fn = lib().song_batch_run_graph if graph else lib().song_batch_run
+        _check(fn(self._h, dp(queries), queries.shape[0], int(pq), int(k), ctypes.byref(qp), dp(ids), dp(dists),
+                  ids.shape[1], dp(flags), dp(stats), ctypes.c_void_p(stream)))
+
+    def counts(self, stream=None):
+        """Queries per tier and the re-run count of the last run (synchronizes)."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        out = np.zeros(len(self.tiers) + 1, dtype=np.uint32)
+        _check(lib().song_batch_counts(self._h, _ptr(out), ctypes.c_void_p(stream)))
+        return out
+
+    def grid(self, t):
+        """(workgroups, waves per group, LDS bytes per wave) of tier t (t = len(tiers): retry)."""
+        g, w, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().song_batch_grid(self._h, t, ctypes.byref(g), ctypes.byref(w), ctypes.byref(l)))
+        return g.value, w.value, l.value
 
 
 # ---------------------------------------------------------------- host library (CPU side)

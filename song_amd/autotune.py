@@ -1,0 +1,134 @@
+"""Autotuner of the batching form: per-tier launch knobs picked by measurement.
+
+For one index, storage, mode and workload (device queries with per-query k and pq), the
+tuner splits the queries into tiers by pq and tunes each tier on its own queries by
+coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
+
+  waves_per_group  queries in flight per workgroup (1, 2, 4)
+  hash_log2        visited filter size in LDS (smaller: more waves fit, more spills)
+  shape            lanes per candidate row (the sub-wave width of one distance)
+  slack            heap depth beyond pq (tie room; overflows cost a re-run)
+  groups_per_cu    persistent grid size (0 = all that fit)
+
+Each candidate runs the tier's queries through song_batch_run and is timed with events
+(median of `reps`); a candidate is accepted only if every result equals the default
+configuration's (ids, distances and flags), so tuning never changes results.  The
+table is a JSON-able dict that `Batch(index, table["tiers"], ...)` takes directly.
+"""
+from __future__ import annotations
+
+import json
+import time
+
+import numpy as np
+
+from . import MODE_EXACT, STORE_F32, Batch, SongError, tier
+
+SHAPES = (0, 1, 2, 3, 4)
+
+
+def default_tiers(pqs):
+    """One tier per distinct pq value (ascending)."""
+    return [tier(int(p)) for p in sorted(set(int(x) for x in pqs))]
+
+
+def _time_run(batch, q, ids, flags, pq_arr, k_arr, dists, reps, entry=None, steps=None):
+    import torch
+
+    batch.run(q, ids, flags, pq_arr=pq_arr, k_arr=k_arr, dists=dists, entry=entry, steps=steps)  # warm-up
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        batch.run(q, ids, flags, pq_arr=pq_arr, k_arr=k_arr, dists=dists, entry=entry, steps=steps)
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def _neighbours(value, choices):
+    return [c for c in choices if c != value]
+
+
+def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STORE_F32, rerank=1, reps=3,
+         max_queries_per_tier=20000, k_ld=None, entry=None, steps=None, verbose=False):
+    """Tune every tier on its own queries; returns the table (dict)."""
+    import torch
+
+    t_start = time.time()
+    pq_h = pq_arr.cpu().numpy()
+    tiers = [dict(t) for t in (tiers or default_tiers(pq_h))]
+    bounds = [t["pq_max"] for t in tiers]
+    k_ld = k_ld or int(k_arr.max().item())
+    log = []
+    lo = 0
+    for ti, t in enumerate(tiers):
+        sel = np.nonzero((pq_h > lo) & (pq_h <= t["pq_max"]))[0][:max_queries_per_tier]
+        lo = t["pq_max"]
+        if sel.size == 0:
+            continue
+        idx = torch.from_numpy(sel.astype(np.int64)).to(queries.device)
+        q = queries[idx].contiguous()
+        pa, ka = pq_arr[idx].contiguous(), k_arr[idx].contiguous()
+        en = entry[idx].contiguous() if entry is not None else None
+        st = steps[idx].contiguous() if steps is not None else None
+        nq = q.shape[0]
+        ids = torch.empty((nq, k_ld), dtype=torch.int32, device=q.device)
+        dists = torch.empty((nq, k_ld), dtype=torch.float32, device=q.device)
+        flags = torch.empty(nq, dtype=torch.int32, device=q.device)
+
+        def measure(cfg, want=None):
+            try:
+                b = Batch(index, [cfg], nq, mode=mode, storage=storage, rerank=rerank)
+            except SongError as e:  # e.g. LDS beyond the limit
+                return None, str(e)
+            ids.fill_(-7)
+            ms = _time_run(b, q, ids, flags, pa, ka, dists, reps, en, st)
+            out = (ids.clone(), dists.clone(), flags.clone())
+            b.close()
+            if want is not None:
+                same = all(torch.equal(a, w) for a, w in zip(out, want))
+                if not same:
+                    return None, "results differ"
+            return ms, out
+
+        best = dict(t)
+        best_ms, ref = measure(best)
+        if best_ms is None:
+            raise SongError(f"tier {ti}: default configuration failed: {ref}")
+        trials = [(dict(best), best_ms)]
+        for key, choices in (("waves_per_group", (1, 2, 4)),
+                             ("hash_log2", tuple(range(8, 15))),
+                             ("shape", SHAPES),
+                             ("slack", (0, 8, 32, 128)),
+                             ("groups_per_cu", (0, 1, 2, 4, 8))):
+            cur = best[key]
+            if key == "hash_log2" and cur == 0:
+                cur = -1
+            for v in _neighbours(cur, choices):
+                cand = dict(best)
+                cand[key] = v
+                ms, _ = measure(cand, want=ref)
+                trials.append((dict(cand), ms))
+                if ms is not None and ms < best_ms:
+                    best, best_ms = cand, ms
+        tiers[ti] = best
+        entry_log = {"tier": ti, "pq_max": t["pq_max"], "queries": int(nq), "ms": best_ms,
+                     "qps": nq / (best_ms * 1e-3), "config": best,
+                     "trials": [{"config": c, "ms": m} for c, m in trials]}
+        log.append(entry_log)
+        if verbose:
+            print(f"[autotune] tier {ti} pq<={t['pq_max']}: {nq} queries, {best_ms:.2f} ms, {best}", flush=True)
+    return {"metric": index.metric, "n": index.n, "dim": index.dim, "mode": mode, "storage": storage,
+            "bounds": bounds, "tiers": tiers, "log": log, "seconds": time.time() - t_start}
+
+
+def save(table, path):
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1)
+
+
+def load(path):
+    with open(path) as f:
+        return json.load(f)

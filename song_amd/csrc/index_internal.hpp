@@ -1,0 +1,68 @@
+// Internal interface shared by the C ABI sources (song_api.hip, batch.hip): the index
+// object, error reporting and launch planning.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/song_amd.h"
+#include "search_inst.hpp"
+
+namespace song {
+
+// record the calling thread's last error text; returns code
+int fail(int code, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(SONG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace song
+
+struct song_index {
+    int device = 0;
+    uint64_t n = 0;
+    int dim = 0, ld = 0, metric = 0, shift = 0, gstride = 0;
+    float *d_data = nullptr;
+    uint32_t *d_graph = nullptr;
+    float *d_norms = nullptr;
+    int integer_valued = 0;
+    float vmin = 0.f, vmax = 0.f;
+    size_t max_lds = 65536;
+    // visited spill tables, one per launch slot, one set per stream: calls on one stream
+    // run in order; calls on different streams use different tables
+    struct Spill {
+        uint32_t *d = nullptr;
+        size_t slots = 0;
+    };
+    std::unordered_map<void *, Spill> spill;
+    std::mutex spill_mu;
+    uint8_t *d_lp[3] = {nullptr, nullptr, nullptr};  // low-precision rows by SONG_STORE_*
+    float *d_scale[3] = {nullptr, nullptr, nullptr};
+    size_t lp_stride[3] = {0, 0, 0};
+};
+
+namespace song {
+
+// launch plan of one search configuration
+struct Plan {
+    bool exact;
+    int store, shape, hlog, slack, qcap, qld;
+    size_t lds;  // LDS bytes of one wave
+};
+
+size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld);
+// shape < 0: by row width; otherwise the given lanes-per-row shape
+int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape = -1);
+int default_hlog(int pq);
+SearchFn search_fn(int metric, const Plan &p, bool persistent = false);
+// integer test and range of a device float array (synchronizes st)
+int device_range(const float *d, uint64_t count, int *integer, float *vmin, float *vmax, hipStream_t st);
+bool fp32_exact(const song_index *ix, int q_integer, float qmin, float qmax);
+
+}  // namespace song

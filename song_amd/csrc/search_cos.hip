@@ -1,6 +1,8 @@
-// search_kernel instantiations for the COS metric (one file per metric so they compile in parallel).
+// search kernel instantiations for the COS metric (one file per metric so they compile in parallel).
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_cos(bool exact, int store, int shape) { return pick_kernel<M_COS>(exact, store, shape); }
+SearchFn search_fn_cos(bool exact, int store, int shape, bool persistent) {
+    return pick_kernel<M_COS>(exact, store, shape, persistent);
+}
 }  // namespace song

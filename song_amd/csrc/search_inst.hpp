@@ -1,4 +1,4 @@
-// Instantiation table for search_kernel<METRIC, EXACT, STORE, SHAPE>.  SHAPE picks the
+// Instantiation table for search_kernel / search_persistent<METRIC, EXACT, STORE, SHAPE>.  SHAPE picks the
 // lanes per candidate row and the 16-byte units per lane from the units per row; EXACT
 // (double accumulation, bit-exact to the CPU) exists for float32 rows only.
 #pragma once
@@ -18,30 +18,36 @@ inline int shape_for_units(int units) {
     return 4;
 }
 
+template <int METRIC, bool EXACT, int STORE, int SHAPE>
+inline SearchFn pick_form(bool persistent) {
+    return persistent ? search_persistent<METRIC, EXACT, STORE, SHAPE> : search_kernel<METRIC, EXACT, STORE, SHAPE>;
+}
+
 template <int METRIC, bool EXACT, int STORE>
-inline SearchFn pick_shape(int shape) {
+inline SearchFn pick_shape(int shape, bool persistent) {
     switch (shape) {
-    case 0: return search_kernel<METRIC, EXACT, STORE, 0>;
-    case 1: return search_kernel<METRIC, EXACT, STORE, 1>;
-    case 2: return search_kernel<METRIC, EXACT, STORE, 2>;
-    case 3: return search_kernel<METRIC, EXACT, STORE, 3>;
-    default: return search_kernel<METRIC, EXACT, STORE, 4>;
+    case 0: return pick_form<METRIC, EXACT, STORE, 0>(persistent);
+    case 1: return pick_form<METRIC, EXACT, STORE, 1>(persistent);
+    case 2: return pick_form<METRIC, EXACT, STORE, 2>(persistent);
+    case 3: return pick_form<METRIC, EXACT, STORE, 3>(persistent);
+    default: return pick_form<METRIC, EXACT, STORE, 4>(persistent);
     }
 }
 
 template <int METRIC>
-inline SearchFn pick_kernel(bool exact, int store, int shape) {
-    if (exact) return pick_shape<METRIC, true, ST_F32>(shape);
+inline SearchFn pick_kernel(bool exact, int store, int shape, bool persistent) {
+    if (exact) return pick_shape<METRIC, true, ST_F32>(shape, persistent);
     switch (store) {
-    case ST_BF16: return pick_shape<METRIC, false, ST_BF16>(shape);
-    case ST_FP8: return pick_shape<METRIC, false, ST_FP8>(shape);
-    default: return pick_shape<METRIC, false, ST_F32>(shape);
+    case ST_BF16: return pick_shape<METRIC, false, ST_BF16>(shape, persistent);
+    case ST_FP8: return pick_shape<METRIC, false, ST_FP8>(shape, persistent);
+    default: return pick_shape<METRIC, false, ST_F32>(shape, persistent);
     }
 }
 
 // defined in search_{l2,ip,cos}.hip
-SearchFn search_fn_l2(bool exact, int store, int shape);
-SearchFn search_fn_ip(bool exact, int store, int shape);
-SearchFn search_fn_cos(bool exact, int store, int shape);
+// persistent: the work-list kernel search_persistent instead of one group per query
+SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent);
+SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent);
+SearchFn search_fn_cos(bool exact, int store, int shape, bool persistent);
 
 }  // namespace song

@@ -69,6 +69,14 @@ struct SearchParams {
     const float *scale;     // per-row scale of data_lp
     size_t lp_stride;
     int rerank;             // low-precision kernels: re-rank the final list in exact FP32
+    // persistent kernel: waves take list positions t = (*work_next)++ while t < *work_count
+    // and search query qidx[t] (t itself without a list)
+    uint32_t *work_next;
+    const uint32_t *work_count;
+    uint32_t wave_lds;      // LDS bytes per wave (persistent kernel, several waves per group)
+    // queries whose visited set or queue overflowed are appended here for a re-run
+    uint32_t *retry_list;
+    uint32_t *retry_count;
 };
 
 // ---------------------------------------------------------------- keys
@@ -134,6 +142,7 @@ template <> struct KeySel<true> { using T = KeyD; };
 
 // ---------------------------------------------------------------- wave helpers
 
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lanes_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -266,7 +275,7 @@ template <int METRIC, bool EXACT, int STORE, int SUB, int NVU, class Key>
 __device__ __forceinline__ Key eval_candidates(const SearchParams &P, int m, uint32_t cand, const uint32_t *cid,
                                                const float *qv, float lenv) {
     constexpr int EPU = Store<STORE>::EPU;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     constexpr int G = 64 / SUB;
     const int g = lane / SUB, sl = lane % SUB;
     const int nu = (STORE == ST_F32) ? (P.ld >> 2) : (int)(P.lp_stride >> 4);
@@ -389,7 +398,7 @@ __device__ __forceinline__ KeyF wave_min(KeyF k) {
 // 64 equal chunks (one LDS read per lane) and keeps the chunk holding the boundary.
 template <class Key, class Pred>
 __device__ __forceinline__ int prefix_count(const Key *arr, int n, Pred below) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     int lo = 0, len = n;
     while (len > 64) {
         const int stride = (len + 63) >> 6;
@@ -416,17 +425,17 @@ template <> struct Shape<4> { static constexpr int SUB = 64, NV = 16; };  // wid
 
 // ---------------------------------------------------------------- the search kernel
 
+// One query on one wave: qi is the query, slot_q the wave's spill table, smem its LDS.
 template <int METRIC, bool EXACT, int STORE, int SHAPE>
-__global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
+__device__ __forceinline__ void search_query(const SearchParams &P, const int qi, const int slot_q,
+                                             unsigned char *smem) {
     using Key = typename KeySel<EXACT>::T;
     constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
-    extern __shared__ __align__(16) unsigned char smem[];
-
-    const int lane = threadIdx.x;
-    const int slot_q = (int)blockIdx.x;
-    const int qi = P.qidx ? (int)P.qidx[P.qbase + slot_q] : P.qbase + slot_q;
-    const int pq = P.pq_arr ? P.pq_arr[qi] : P.pq;
-    const int k = P.k_arr ? min(P.k_arr[qi], pq) : min(P.k, pq);
+    const int lane = lane_id();
+    lds_fence();
+    // per-query budgets are clamped to the launch's LDS sizing (pq) and output rows (out_ld)
+    const int pq = P.pq_arr ? max(1, min(P.pq_arr[qi], P.pq)) : P.pq;
+    const int k = max(0, min(min(P.k_arr ? P.k_arr[qi] : P.k, pq), P.out_ld));
     const int capq = pq + P.slack;
     const uint32_t H = 1u << P.hlog;
     const uint32_t hmask = H - 1u;
@@ -464,9 +473,15 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     // start vertices (graph.h:175-181: vertex 0; the learned form passes its own), each
     // entered once through the visited set and placed in the queue in (dist, id) order
     {
+        // entries >= n (0xFFFFFFFF pads shorter lists) are skipped; none left means vertex 0
         const int ne = P.entry ? P.n_entry : 1;
-        const uint32_t sid = (lane < ne) ? (P.entry ? P.entry[(size_t)qi * ne + lane] : 0u) : 0u;
-        const bool fresh = lane < ne && vis_insert(vis, hmask, P.hlog, sid);
+        uint32_t sid = (lane < ne) ? (P.entry ? P.entry[(size_t)qi * ne + lane] : 0u) : 0xFFFFFFFFu;
+        bool use = lane < ne && (uint64_t)sid < P.n;
+        if (ballot(use) == 0ull) {
+            use = lane == 0;
+            sid = 0u;
+        }
+        const bool fresh = use && vis_insert(vis, hmask, P.hlog, sid);
         const uint64_t fm = ballot(fresh);
         const int m0 = __popcll(fm);
         if (fresh) cid[lanes_below(fm)] = sid;
@@ -676,6 +691,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
         if (P.out_dist) P.out_dist[ob + i] = ok ? e.dist() : __int_as_float(0x7f800000);
     }
     if (lane == 0) {
+        if (P.retry_list && (flags & (F_VIS_OVF | F_Q_OVF))) P.retry_list[atomicAdd(P.retry_count, 1u)] = (uint32_t)qi;
         if (P.out_flags) P.out_flags[qi] = flags;
         if (P.out_stats) {
             uint32_t *o = P.out_stats + 8 * (size_t)qi;
@@ -688,6 +704,38 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
             o[6] = 0u;
             o[7] = 0u;
         }
+    }
+}
+
+// One workgroup (one wave) per query of the launch.
+template <int METRIC, bool EXACT, int STORE, int SHAPE>
+__global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int slot_q = (int)blockIdx.x;
+    const int qi = P.qidx ? (int)P.qidx[P.qbase + slot_q] : P.qbase + slot_q;
+    search_query<METRIC, EXACT, STORE, SHAPE>(P, qi, slot_q, smem);
+}
+
+// Persistent grid: every wave takes queries from the shared list until it is exhausted
+// (each wave reaches the exit once *work_next passes *work_count).  A workgroup holds
+// blockDim.x / 64 waves, each on its own wave_lds-byte slice of LDS and its own spill table.
+template <int METRIC, bool EXACT, int STORE, int SHAPE>
+__global__ __launch_bounds__(256) void search_persistent(SearchParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = (int)(threadIdx.x >> 6);
+    const int slot = (int)blockIdx.x * (int)(blockDim.x >> 6) + w;
+    unsigned char *mine = smem + (size_t)w * P.wave_lds;
+    const uint32_t count = *P.work_count;
+    for (;;) {
+        // Every lane issues the add (lane 0 adds 1, the others 0) and lane 0's old value is
+        // broadcast.  No lane-0 branch here: the query's lane-0 epilogue ends the previous
+        // iteration, and a second branch on the same predicate lets the compiler thread the
+        // back edge so that lanes 1..63 start the next iteration without lane 0.
+        const uint32_t v = atomicAdd(P.work_next, lane_id() == 0 ? 1u : 0u);
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+        if (t >= count) break;
+        const int qi = P.qidx ? (int)P.qidx[t] : (int)t;
+        search_query<METRIC, EXACT, STORE, SHAPE>(P, qi, slot, mine);
     }
 }
 

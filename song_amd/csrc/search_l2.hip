@@ -1,6 +1,8 @@
-// search_kernel instantiations for the L2 metric (one file per metric so they compile in parallel).
+// search kernel instantiations for the L2 metric (one file per metric so they compile in parallel).
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_l2(bool exact, int store, int shape) { return pick_kernel<M_L2>(exact, store, shape); }
+SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent) {
+    return pick_kernel<M_L2>(exact, store, shape, persistent);
+}
 }  // namespace song

@@ -13,14 +13,15 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/song_amd.h"
-#include "search_inst.hpp"
+#include "index_internal.hpp"
 
 using namespace song;
 
+namespace song {
+
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...) {
+int fail(int code, const char *fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -30,34 +31,7 @@ static int fail(int code, const char *fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) return fail(SONG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-struct song_index {
-    int device = 0;
-    uint64_t n = 0;
-    int dim = 0, ld = 0, metric = 0, shift = 0, gstride = 0;
-    float *d_data = nullptr;
-    uint32_t *d_graph = nullptr;
-    float *d_norms = nullptr;
-    int integer_valued = 0;
-    float vmin = 0.f, vmax = 0.f;
-    size_t max_lds = 65536;
-    // visited spill tables, one per launch slot, one set per stream: calls on one stream
-    // run in order; calls on different streams use different tables
-    struct Spill {
-        uint32_t *d = nullptr;
-        size_t slots = 0;
-    };
-    std::unordered_map<void *, Spill> spill;
-    std::mutex spill_mu;
-    uint8_t *d_lp[3] = {nullptr, nullptr, nullptr};  // low-precision rows by SONG_STORE_*
-    float *d_scale[3] = {nullptr, nullptr, nullptr};
-    size_t lp_stride[3] = {0, 0, 0};
-};
+}  // namespace song
 
 // visited spill tables: 2^13 u32 (32 KB) per launch slot, launches of at most kChunk queries
 static constexpr int kGhlog = 13;
@@ -81,7 +55,7 @@ static int ensure_gvis(song_index *ix, void *stream, size_t slots, uint32_t **ou
     return SONG_OK;
 }
 
-extern "C" const char *song_last_error(void) { return g_err.c_str(); }
+extern "C" const char *song_last_error(void) { return song::g_err.c_str(); }
 
 extern "C" void song_search_opts_default(song_search_opts *o) {
     o->mode = SONG_MODE_AUTO;
@@ -175,7 +149,7 @@ static float unord_host(unsigned o) {
     return f;
 }
 
-static int device_range(const float *d, uint64_t count, int *integer, float *vmin, float *vmax, hipStream_t st) {
+int song::device_range(const float *d, uint64_t count, int *integer, float *vmin, float *vmax, hipStream_t st) {
     unsigned init[3] = {0u, 0xffffffffu, 0u}, res[3];
     unsigned *d_out = nullptr;
     HIP_TRY(hipMalloc(&d_out, sizeof(init)));
@@ -292,14 +266,14 @@ extern "C" int song_index_get_storage(const song_index *ix, int store, void *cod
 extern "C" int song_index_destroy(song_index *ix) {
     if (!ix) return SONG_OK;
     for (int i = 0; i < 3; ++i) {
-        if (ix->d_lp[i]) hipFree(ix->d_lp[i]);
-        if (ix->d_scale[i]) hipFree(ix->d_scale[i]);
+        if (ix->d_lp[i]) (void)hipFree(ix->d_lp[i]);
+        if (ix->d_scale[i]) (void)hipFree(ix->d_scale[i]);
     }
-    if (ix->d_data) hipFree(ix->d_data);
-    if (ix->d_graph) hipFree(ix->d_graph);
-    if (ix->d_norms) hipFree(ix->d_norms);
+    if (ix->d_data) (void)hipFree(ix->d_data);
+    if (ix->d_graph) (void)hipFree(ix->d_graph);
+    if (ix->d_norms) (void)hipFree(ix->d_norms);
     for (auto &kv : ix->spill)
-        if (kv.second.d) hipFree(kv.second.d);
+        if (kv.second.d) (void)hipFree(kv.second.d);
     delete ix;
     return SONG_OK;
 }
@@ -308,24 +282,18 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 // ---------------------------------------------------------------- launch planning
 
-struct Plan {
-    bool exact;
-    int store, shape, hlog, slack, qcap, qld;
-    size_t lds;
-};
-
-static size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld) {
+size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld) {
     size_t key = exact ? 16 : 8;
     return ((size_t)4 << hlog) + (size_t)qcap * key + (size_t)((pq + 1) & ~1) * key + 64 * 4 + (size_t)qld * 4;
 }
 
-static int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p) {
+int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
     if (store != SONG_STORE_F32 && !ix->d_lp[store]) return fail(SONG_E_ARG, "storage %d not built", store);
     p->exact = exact && store == SONG_STORE_F32;
     p->store = store;
     const int units = store == SONG_STORE_F32 ? ix->ld / 4 : (int)(ix->lp_stride[store] / 16);
-    p->shape = shape_for_units(units);
+    p->shape = shape >= 0 ? std::min(shape, 4) : shape_for_units(units);
     p->hlog = hlog;
     p->slack = slack;
     // queue buffer: pq + slack live entries, 64 of merge room, 64 of head room
@@ -339,14 +307,14 @@ static int make_plan(const song_index *ix, bool exact, int store, int pq, int sl
 
 // 4096 slots (16 KB) hold the ~1-3k distinct vertices a search discovers up to pq ~ 1000
 // at the 3/4 load limit; queries that discover more are re-run with larger tables.
-static int default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
+int song::default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
 
-static SearchFn search_fn(int metric, const Plan &p) {
+SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
     switch (metric) {
-    case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape);
-    case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape);
-    default: return search_fn_cos(p.exact, p.store, p.shape);
+    case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape, persistent);
+    case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape, persistent);
+    default: return search_fn_cos(p.exact, p.store, p.shape, persistent);
     }
 }
 
@@ -375,7 +343,7 @@ static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned
 // FP32 accumulation is exact when every product and every partial sum is an integer
 // below 2^24: integer-valued data and queries with dim * span^2 < 2^24 (L2) or
 // dim * max|x| * max|q| < 2^24 (IP).
-static bool fp32_exact(const song_index *ix, int q_integer, float qmin, float qmax) {
+bool song::fp32_exact(const song_index *ix, int q_integer, float qmin, float qmax) {
     if (!ix->integer_valued || !q_integer || ix->metric == SONG_COS) return false;
     double lim = 16777216.0;
     if (ix->metric == SONG_L2) {
@@ -477,7 +445,7 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
     sp.max_steps = opts.max_steps;
     sp.rerank = opts.rerank;
     if (nrun == 0) {
-        if (tmp_flags) hipFree(tmp_flags);
+        if (tmp_flags) (void)hipFree(tmp_flags);
         return SONG_OK;
     }
     rc = launch(ix, plan, sp, (unsigned)nrun, st, gvis);
@@ -531,9 +499,9 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
             rc = launch(ix, plan, sp2, (unsigned)redo.size(), st, gvis);
             if (rc != SONG_OK) break;
         }
-        if (d_idx) hipFree(d_idx);
+        if (d_idx) (void)hipFree(d_idx);
     }
-    if (tmp_flags) hipFree(tmp_flags);
+    if (tmp_flags) (void)hipFree(tmp_flags);
     return rc;
 }
 
@@ -567,10 +535,10 @@ extern "C" int song_search(song_index *ix, const float *queries, uint64_t nq, in
             hipMemcpy(out_flags, d_f, nq * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(SONG_E_HIP, "download flags");
     }
-    if (d_q) hipFree(d_q);
-    if (d_i) hipFree(d_i);
-    if (d_d) hipFree(d_d);
-    if (d_f) hipFree(d_f);
+    if (d_q) (void)hipFree(d_q);
+    if (d_i) (void)hipFree(d_i);
+    if (d_d) (void)hipFree(d_d);
+    if (d_f) (void)hipFree(d_f);
     return rc;
 }
 

@@ -1,0 +1,82 @@
+"""Structure of the compiled persistent search kernels (gfx950 assembly of this build).
+
+The persistent kernel's loop over work-list positions must stay wave-uniform: one loop
+at depth 1 whose head takes the next position with one atomic, broadcasts it
+(readfirstlane) and leaves on a scalar compare.  A lane-0-only branch at the head let
+the compiler thread the back edge from the query's lane-0 epilogue, so that lanes 1..63
+started the next iteration without lane 0 (and re-read position 0 for ever); this test
+fails on that shape.
+"""
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "song_amd", "csrc")
+HIPCC = "/opt/rocm/bin/hipcc"
+
+
+def _asm(metric, out_dir):
+    out = os.path.join(out_dir, f"{metric}.s")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-I" + os.path.join(ROOT, "include"), "--cuda-device-only", "-S",
+           os.path.join(CSRC, f"search_{metric}.hip"), "-o", out]
+    return subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE), out
+
+
+def _functions(text, prefix):
+    out, cur, name = {}, None, None
+    for line in text.split("\n"):
+        m = re.match(r"^(" + prefix + r"\w+):", line)
+        if m:
+            name, cur = m.group(1), []
+            continue
+        if cur is not None:
+            if line.startswith(".Lfunc_end"):
+                out[name], cur = cur, None
+            else:
+                cur.append(line)
+    return out
+
+
+def loop_head_is_uniform(body):
+    """True if the work loop is one depth-1 loop: atomic, readfirstlane, scalar compare,
+    and no exec-narrowing latch in front of its header."""
+    d1 = [k for k, x in enumerate(body) if "This Loop Header: Depth=1" in x]
+    at = [k for k, x in enumerate(body) if "global_atomic_add" in x]
+    if len(d1) != 1 or not at or at[0] < d1[0]:
+        return False
+    seg = "\n".join(body[at[0]:at[0] + 25])
+    if "v_readfirstlane_b32" not in seg or "s_cmp_ge_u32" not in seg:
+        return False
+    return "s_andn2_b64 exec, exec" not in "\n".join(body[max(0, d1[0] - 8):d1[0]])
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_persistent_loop_is_wave_uniform():
+    tmp = tempfile.mkdtemp(prefix="song_asm_")
+    try:
+        jobs = [_asm(m, tmp) for m in ("l2", "ip", "cos")]
+        kernels = {}
+        for p, out in jobs:
+            _, err = p.communicate(timeout=600)
+            assert p.returncode == 0, err.decode(errors="replace")[-2000:]
+            kernels.update(_functions(open(out).read(), r"_ZN4song17search_persistent"))
+        assert len(kernels) == 60
+        bad = [k for k, body in kernels.items() if not loop_head_is_uniform(body)]
+        assert not bad, f"{len(bad)} persistent kernels with a split work loop, e.g. {bad[:3]}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def test_checker_flags_a_split_loop():
+    """The shape check itself: a latch that narrows exec before the header is rejected."""
+    good = ["s_branch .LBB0_3", ".LBB0_3:  ; =>This Loop Header: Depth=1", "global_atomic_add v1, v3, v1, s[4:5] sc0",
+            "v_readfirstlane_b32 s0, v1", "s_cmp_ge_u32 s58, s0"]
+    assert loop_head_is_uniform(good)
+    split = ["s_andn2_b64 exec, exec, s[0:1]", "s_cbranch_execz .LBB0_298"] + good[1:3] + ["s_nop 0"] * 30 + good[3:]
+    assert not loop_head_is_uniform(split)
