@@ -1,6 +1,6 @@
 #!/usr/bin/env python
 """Benchmark: QPS at recall@10 >= 0.95 on the 1M x 128 L2 workload, one GPU,
-learned + BF16 form against the plain form.
+learned + BF16 batched form against the plain form.
 
 Workload: 1,000,000 base vectors, dim 128, integer-valued mixture of 1,000
 Gaussians (seed 2), 10,000 queries from the same mixture, L2, degree-31 graph
@@ -8,14 +8,23 @@ from the reference insertion build (byte-identical, built by the parallel
 builder), top-10.  A "step" is one search of the whole 10,000-query batch with
 queries, index and outputs resident in HBM.
 
-Plain form: the bit-exact search from vertex 0; the smallest pq of the sweep
-whose recall@10 (tie-aware, against a float64 brute force on the GPU) reaches
-the target is timed.  Learned + BF16 form: BF16 rows, MLP-predicted start
-vertices and stop caps (trained here on 200,000 trace-labelled queries drawn
-from the same mixture, disjoint from the benchmark queries), confidence
-fallback to the plain full-budget search; the fastest (pq, stop scale) setting
-whose recall reaches the target is timed.  `value` is the learned + BF16 QPS.
-Prints one JSON line.
+Forms timed:
+  plain          the bit-exact search from vertex 0, one launch per batch (one
+                 workgroup per query); the smallest pq of the sweep whose recall@10
+                 (tie-aware, against a float64 brute force on the GPU) reaches the
+                 target.  The ratio's denominator.
+  plain batched  the same pq through the batching form (autotuned persistent
+                 kernel, hipGraph replay); ids checked equal to the plain form's.
+  learned+BF16   BF16 rows with exact FP32 re-rank, MLP start vertices and stop
+  batched        caps (trained here on 200,000 trace-labelled queries from the same
+                 mixture, disjoint from the benchmark queries), confidence fallback
+                 to the plain full-budget search; every query in one batched search
+                 (per-query pq / entries / caps made on the GPU, autotuned tiers),
+                 the whole pipeline -- features, MLP, plan, search -- one graph
+                 replay per step.  `value` is its QPS at the fastest setting whose
+                 recall reaches the target.
+`--workload c4` measures the batching form on its own workload instead (2M x 96
+inner product, 100,000 queries with uneven k and pq).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -32,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_search_l2_c2_1m.json")
 
 
 def log(*a):
@@ -61,38 +71,76 @@ def timed(fn, reps, torch):
     return float(np.median(times))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--target-recall", type=float, default=0.95)
-    ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--pqs", default="256,384,512,640,768,896,1024")
-    ap.add_argument("--pq", type=int, default=0, help="plain form: skip the sweep and time this pq")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="plain form only")
-    ap.add_argument("--train-queries", type=int, default=200_000)
-    ap.add_argument("--learned-pqs", default="64,96,128,192,256,384")
-    ap.add_argument("--step-scales", default="1.5,2.5,0")
-    ap.add_argument("--no-recall", action="store_true",
-                    help="with --pq: skip the ground truth and recall (profiling runs)")
-    ap.add_argument("--copy-mb", type=int, default=0,
-                    help="run one device copy of this many MiB first (a known-size kernel that "
-                         "calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries shard across GPUs "
-                                   "with no communication, see DESIGN.md)"}))
-        sys.exit(2)
-    if args.no_recall and not args.pq:
-        ap.error("--no-recall needs --pq")
+def timed_steps(fn, steps, warmup, torch, stream=None):
+    """W untimed calls, then K calls between synchronizations; ms per call from events on
+    `stream` (the stream the work is launched on) and from the host clock."""
+    stream = stream or torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return e0.elapsed_time(e1) / steps, wall
 
-    import torch
 
-    import song_amd
-    from song_amd import datasets, groundtruth, workloads
+def alg_bytes(st, dim, row_bytes, nq):
+    """Algorithmic bytes of one search launch from per-query counters: every distance
+    reads one row, every expansion reads (1 + degree) u32 adjacency words, every query
+    reads its vector."""
+    st64 = st.astype(np.int64)
+    return int(st64[:, 1].sum() * row_bytes + st64[:, 2].sum() * 4 + nq * dim * 4)
+
+
+def load_traffic(pq):
+    """HBM bytes per launch of the plain kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) when it was
+    measured at this pq; else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t if t.get("pq") == pq else None
+
+
+def cpu_baseline(base, graph, metric, query, groups, seconds, check=None):
+    """Oracle restatement of graph.h:169-216 on the host cores over a bounded sample.
+    groups: list of (pq, query index array); the sample takes the same share of each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    threads = min(16, os.cpu_count() or 1)
+    probe = []
+    for pq, idx in groups:
+        tp = time.perf_counter()
+        pyoracle.search_batch(base, graph, metric, query[idx[:8]], pq, threads=1)
+        probe.append((time.perf_counter() - tp) / min(8, len(idx)))
+    total_q = sum(len(idx) for _, idx in groups)
+    mean_t = sum(p * len(idx) for p, (_, idx) in zip(probe, groups)) / total_q
+    frac = min(1.0, seconds * threads / max(mean_t * total_q, 1e-9))
+    done, tcpu, same = 0, 0.0, True
+    for pq, idx in groups:
+        take = idx[:max(threads, int(len(idx) * frac))]
+        tc = time.perf_counter()
+        cids, _ = pyoracle.search_batch(base, graph, metric, query[take], pq, threads=threads)
+        tcpu += time.perf_counter() - tc
+        done += len(take)
+        if check is not None:
+            same &= check(pq, take, cids)
+    return {"value": round(done / tcpu, 1), "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"{done} of the {total_q} queries (same share of every pq group), oracle restatement of "
+                      f"graph.h:169-216 on {threads} threads, static split; single-thread probe "
+                      f"{1.0 / mean_t:.1f} q/s", "cpu_model": cpu_model(), "ids_equal_gpu": bool(same)}
+
+
+def bench_c2(args, torch, song_amd):
+    from song_amd import autotune, datasets, groundtruth, workloads
 
     dev = torch.device("cuda:0")
     t0 = time.time()
@@ -127,7 +175,7 @@ def main():
     stats = torch.empty((nq, 8), dtype=torch.int32, device=dev)
     ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
     # integer-valued data and queries: FP32 accumulation is exact (same ids as the exact mode)
-    mode = song_amd.MODE_FP32 if ix.integer_valued and bool(np.all(np.rint(query) == query)) else song_amd.MODE_EXACT
+    mode = song_amd.resolve_mode(ix, q)
 
     def plain(pq, st=None, retry=0):
         ix.search_device(q, pq, k, ids, None, flags, st, mode=mode, retry=retry)
@@ -157,31 +205,44 @@ def main():
         log(f"plain form did not reach the target recall; timing pq={pq}")
     else:
         pq, r, st = chosen
+    plain(pq, None, retry=1)
+    torch.cuda.synchronize()
+    plain_ids = ids.clone()
     retry = 1 if int((flags.cpu().numpy() & 3).sum()) > 0 else 0
 
-    # timed region of the plain form: W untimed steps, then K steps between synchronizations
-    for _ in range(args.warmup):
-        plain(pq, retry=retry)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.steps):
-        plain(pq, retry=retry)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms_plain = ev0.elapsed_time(ev1) / args.steps
+    ms_plain, wall_plain = timed_steps(lambda: plain(pq, retry=retry), args.steps, args.warmup, torch, stream)
     qps_plain = nq / (ms_plain / 1e3)
     plain_info = {"pq": pq, "recall@10": round(r, 4), "qps": round(qps_plain, 1), "ms_per_step": round(ms_plain, 4),
-                  "mean_expansions": round(float(st[:, 0].mean()), 1),
+                  "wall_ms_per_step": round(wall_plain, 4), "mean_expansions": round(float(st[:, 0].mean()), 1),
                   "mean_distances": round(float(st[:, 1].mean()), 1), "arithmetic": "fp32 (exact on integer data)"}
     log(f"plain form: {qps_plain:.0f} q/s at pq={pq} recall={r:.4f}")
+    alg_plain = alg_bytes(st, dim, dim * 4, nq)
 
-    # algorithmic bytes of one plain launch: every distance reads one fp32 row, every
-    # expansion reads (1 + degree) u32 adjacency words, every query reads its own vector
-    st64 = st.astype(np.int64)
-    alg_plain = int(st64[:, 1].sum() * dim * 4 + st64[:, 2].sum() * 4 + nq * dim * 4)
+    # ---------------------------------------------------------------- plain, batched
+    batched_info = None
+    if not args.no_batch:
+        pq_all = torch.full((nq,), pq, dtype=torch.int32, device=dev)
+        k_all = torch.full((nq,), k, dtype=torch.int32, device=dev)
+        tb = time.time()
+        table = autotune.tune(ix, q, pq_all, k_all, mode=mode, reps=3, max_queries_per_tier=nq, k_ld=k)
+        log(f"plain batched: autotuned in {time.time() - tb:.1f}s: {table['tiers']}")
+        b = song_amd.Batch(ix, table["tiers"], nq, mode=mode)
+        side = torch.cuda.Stream()
+        bids = torch.empty_like(ids)
+        bfl = torch.empty_like(flags)
+        side.wait_stream(stream)
+        b.run(q, bids, bfl, pq=pq, k=k, graph=True, stream=side.cuda_stream)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(bids, plain_ids))
+        ms_b, wall_b = timed_steps(lambda: b.run(q, bids, bfl, pq=pq, k=k, graph=True, stream=side.cuda_stream),
+                                   args.steps, args.warmup, torch, side)
+        batched_info = {"pq": pq, "qps": round(nq / (ms_b / 1e3), 1), "ms_per_step": round(ms_b, 4),
+                        "wall_ms_per_step": round(wall_b, 4), "ids_equal_plain": same, "tiers": table["tiers"],
+                        "grid": [b.grid(t) for t in range(len(table["tiers"]) + 1)]}
+        log(f"plain batched: {batched_info['qps']:.0f} q/s, ids equal plain: {same}")
+        b.close()
 
-    # ---------------------------------------------------------------- learned + BF16 form
+    # ---------------------------------------------------------------- learned + BF16, batched
     learned_info = None
     if not args.no_learned and not args.no_recall:
         from song_amd import learned
@@ -194,85 +255,79 @@ def main():
         log(f"learned predictor trained in {time.time() - t2:.1f}s: top1 acc {fit['train_top1_acc']:.3f} "
             f"mean conf {fit['mean_conf']:.3f} mean settle {fit['mean_settle']:.1f}")
         lst = torch.empty((nq, 8), dtype=torch.int32, device=dev)
-        lids = torch.empty((nq, k), dtype=torch.int32, device=dev)
         lsweep = []
         best = None
         for lpq in [int(x) for x in args.learned_pqs.split(",")]:
             for sc in [float(x) for x in args.step_scales.split(",")]:
                 scale = sc if sc > 0 else 1e6
-
-                def run(lpq=lpq, scale=scale):
-                    ls.search(q, lpq, k, lids, pq_fallback=pq, conf_threshold=0.9, step_scale=scale,
-                              storage=song_amd.STORE_BF16)
-
-                plan = ls.search(q, lpq, k, lids, pq_fallback=pq, conf_threshold=0.9, step_scale=scale,
-                                 storage=song_amd.STORE_BF16, stats=lst)
+                lb = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
+                                          storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32)
+                plan = lb.run(q, stats=lst)
                 torch.cuda.synchronize()
-                rl = recall(lids)
-                ms = timed(run, 3, torch)
+                rl = recall(lb.ids)
+                ms = timed(lambda: lb.run(q), 3, torch)
                 stl = lst.cpu().numpy()
+                fb = float((~plan[3]).float().mean().item())
                 ent = {"pq": lpq, "step_scale": sc, "recall@10": round(rl, 4), "qps": round(nq / (ms / 1e3), 1),
-                       "fallback_frac": round(plan[3].numel() / nq, 4),
-                       "mean_expansions": round(float(stl[:, 0].mean()), 1),
+                       "fallback_frac": round(fb, 4), "mean_expansions": round(float(stl[:, 0].mean()), 1),
                        "mean_distances": round(float(stl[:, 1].mean()), 1)}
                 lsweep.append(ent)
                 log(f"learned+bf16 pq={lpq} scale={sc} recall={rl:.4f} {ms:.2f} ms ({ent['qps']:.0f} q/s) "
-                    f"fallback={ent['fallback_frac']:.3f} exp={ent['mean_expansions']}")
+                    f"fallback={fb:.3f} exp={ent['mean_expansions']}")
                 if rl >= args.target_recall and (best is None or ent["qps"] > best[0]["qps"]):
-                    best = (ent, run, stl)
+                    best = (ent, lpq, scale, stl)
+                lb.batch.close()
         if best is not None:
-            ent, run, stl = best
-            for _ in range(args.warmup):
-                run()
+            ent, lpq, scale, stl = best
+            tt = time.time()
+            lb0 = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
+                                       storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32)
+            p = lb0.run(q)
+            table = autotune.tune(ix, q, p[0], lb0.k_arr, tiers=lb0.tiers, mode=song_amd.MODE_FP32,
+                                  storage=song_amd.STORE_BF16, reps=3, max_queries_per_tier=nq, k_ld=k, entry=p[1],
+                                  steps=p[2])
             torch.cuda.synchronize()
-            t_start = time.perf_counter()
-            for _ in range(args.steps):
-                run()
+            eager_ids = lb0.ids.clone()
+            lb0.batch.close()
+            log(f"learned tiers autotuned in {time.time() - tt:.1f}s: {table['tiers']}")
+            lb = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
+                                      storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, tiers=table["tiers"])
+            lb.capture(q)
+            lb.replay()
             torch.cuda.synchronize()
-            ms_l = (time.perf_counter() - t_start) * 1e3 / args.steps
-            stl64 = stl.astype(np.int64)
-            alg_l = int(stl64[:, 1].sum() * (dim * 2 + 4) + stl64[:, 2].sum() * 4 + nq * dim * 4)
+            same = bool(torch.equal(lb.ids, eager_ids))
+            r_final = recall(lb.ids)
+            ms_l, wall_l = timed_steps(lb.replay, args.steps, args.warmup, torch, stream)
+            alg_l = alg_bytes(stl, dim, dim * 2 + 4, nq)
             fit_out = {kk: vv for kk, vv in fit.items() if kk != "loss"}
             fit_out["final_loss"] = fit["loss"][-1]
             learned_info = dict(ent, qps=round(nq / (ms_l / 1e3), 1), ms_per_step=round(ms_l, 4),
-                                algorithmic_bytes_per_step=alg_l, training=fit_out)
-            log(f"learned+bf16 form: {learned_info['qps']:.0f} q/s ({learned_info['qps'] / qps_plain:.2f}x plain)")
+                                wall_ms_per_step=round(wall_l, 4), recall_graph=round(r_final, 4),
+                                graph_ids_equal_eager=same, algorithmic_bytes_per_step=alg_l, tiers=table["tiers"],
+                                training=fit_out)
+            log(f"learned+bf16 batched form: {learned_info['qps']:.0f} q/s ({learned_info['qps'] / qps_plain:.2f}x "
+                f"plain), recall {r_final:.4f}, graph == eager: {same}")
         learned_info = learned_info or {"error": "no learned setting reached the target recall"}
         learned_info["sweep"] = lsweep
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-
-        threads = min(16, os.cpu_count() or 1)
-        tp = time.perf_counter()
-        pyoracle.search_batch(base, graph, metric, query[:50], pq, threads=1)
-        per_q = (time.perf_counter() - tp) / 50
-        sample = int(min(nq, max(threads * 20, args.cpu_seconds * threads / max(per_q, 1e-6))))
-        tc = time.perf_counter()
-        cids, _ = pyoracle.search_batch(base, graph, metric, query[:sample], pq, threads=threads)
-        tcpu = time.perf_counter() - tc
-        plain(pq, retry=1)
-        torch.cuda.synchronize()
-        same = bool(np.array_equal(cids[:, :k].astype(np.int64), ids[:sample].cpu().numpy().astype(np.int64)))
-        cpu = {"value": round(sample / tcpu, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"first {sample} of the {nq} queries at pq={pq} (oracle restatement of graph.h:169-216, "
-                         f"{threads} threads, static split); single-thread probe {1.0 / per_q:.1f} q/s",
-               "cpu_model": cpu_model(), "ids_equal_gpu": same}
-        log(f"cpu baseline: {cpu['value']} q/s on {threads} threads ({sample} queries)")
+        pids = plain_ids.cpu().numpy().astype(np.int64)
+        cpu = cpu_baseline(base, graph, metric, query, [(pq, np.arange(nq))], args.cpu_seconds,
+                           check=lambda p, take, cids: bool(np.array_equal(cids[:, :k].astype(np.int64), pids[take])))
+        log(f"cpu baseline: {cpu['value']} q/s on {cpu['cores']} threads")
 
     have_learned = bool(learned_info and "qps" in learned_info)
     if have_learned:
         value, ms_step, dtype = learned_info["qps"], learned_info["ms_per_step"], "bf16"
-        form = "learned + BF16 (MLP start vertices and stop caps, BF16 rows, exact FP32 re-rank, confidence fallback)"
-        alg, kern_ms = learned_info["algorithmic_bytes_per_step"], learned_info["ms_per_step"]
+        form = ("learned + BF16, batched (MLP start vertices and stop caps, BF16 rows, exact FP32 re-rank, "
+                "confidence fallback, persistent tiers, one graph replay per step)")
     else:
         value, ms_step, dtype = round(qps_plain, 1), ms_plain, "fp32"
         form = "plain (bit-exact ids)"
-        alg, kern_ms = alg_plain, ms_plain
     achieved_plain = alg_plain / (ms_plain / 1e3)
+    traffic = load_traffic(pq)
     out = {
         "metric": "QPS @ recall@10>=0.95, 1M x128 L2, 1 GPU: learned/BF16 form vs plain",
         "value": value,
@@ -290,16 +345,167 @@ def main():
                                "10,000 queries, top-10",
                    "form": form, "k": k, "target_recall": args.target_recall,
                    "plain": plain_info,
-                   "learned_bf16": learned_info,
+                   "plain_batched": batched_info,
+                   "learned_bf16_batched": learned_info,
                    "learned_vs_plain": round(value / qps_plain, 3) if have_learned else None,
                    "plain_sweep": sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved_plain / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK, 5), "traffic": None,
-                     "kernel": "search_kernel<L2, fp32, F32 rows> (plain form)",
+                     "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK, 5),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "kernel": "search_kernel<L2, fp32, F32 rows, shape 1> (plain form, one launch per step)",
+                     "kernel_ms": round(ms_plain, 4),
                      "algorithmic_bytes_per_launch": alg_plain,
-                     "learned_achieved_GBps": round(alg / (kern_ms / 1e3) / 1e9, 2)},
+                     "learned_achieved_GBps": (round(learned_info["algorithmic_bytes_per_step"]
+                                                     / (learned_info["ms_per_step"] / 1e3) / 1e9, 2)
+                                               if have_learned else None)},
         "cpu_baseline": cpu,
     }
+    return out
+
+
+def bench_c4(args, torch, song_amd):
+    """Batching form on its workload: 2M x 96 unit-norm Gaussian rows, inner product,
+    100,000 queries with k from {1, 10, 100} and pq from {32, 64, 128, 256, 512} (k capped at
+    pq), FP32 accumulation.  Timed: the batched search (autotuned tiers, graph replay)
+    against the same queries searched with one plain call per distinct pq."""
+    from song_amd import autotune, datasets, groundtruth, workloads
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    base, query = datasets.normalized_ip(n_base=args.c4_n, n_query=args.c4_nq)
+    graph, bst = workloads.graph_for(base, "ip", verbose=True)
+    log(f"c4 workload ready in {time.time() - t0:.1f}s (graph cached={bst.get('cached')})")
+    ix = song_amd.Index(base, graph, "ip")
+    nq, dim = query.shape
+    k_h, pq_h = datasets.hetero_budgets(nq)
+    q = torch.from_numpy(query).to(dev)
+    k_d, pq_d = torch.from_numpy(k_h).to(dev), torch.from_numpy(pq_h).to(dev)
+    kmax = int(k_h.max())
+    mode = song_amd.MODE_FP32
+    pq_values = sorted(set(pq_h.tolist()))
+    groups = [(p, np.nonzero(pq_h == p)[0]) for p in pq_values]
+
+    # plain: one call per distinct pq over its queries (what the plain API needs)
+    pids = torch.full((nq, kmax), -1, dtype=torch.int32, device=dev)
+    pfl = torch.zeros(nq, dtype=torch.int32, device=dev)
+    gq = [(p, torch.from_numpy(idx).to(dev), q[torch.from_numpy(idx).to(dev)].contiguous()) for p, idx in groups]
+    outs = [(torch.empty((len(i), min(p, kmax)), dtype=torch.int32, device=dev),
+             torch.empty(len(i), dtype=torch.int32, device=dev)) for p, i, _ in gq]
+
+    def plain_all():
+        for (p, _, qq), (o, f) in zip(gq, outs):
+            ix.search_device(qq, p, min(p, kmax), o, None, f, mode=mode, retry=0)
+
+    plain_all()
+    torch.cuda.synchronize()
+    for (p, i, _), (o, f) in zip(gq, outs):
+        pids[i, :o.shape[1]] = o
+        pfl[i] = f
+    ms_plain, _ = timed_steps(plain_all, args.steps, args.warmup, torch)
+
+    # batched, default tiers then autotuned tiers
+    stats = torch.empty((nq, 8), dtype=torch.int32, device=dev)
+    bids = torch.full((nq, kmax), -1, dtype=torch.int32, device=dev)
+    bfl = torch.empty(nq, dtype=torch.int32, device=dev)
+    side = torch.cuda.Stream()
+    results = {}
+    default = autotune.default_tiers(pq_h)
+    tt = time.time()
+    table = autotune.tune(ix, q, pq_d, k_d, mode=mode, reps=3, max_queries_per_tier=20000, k_ld=kmax)
+    log(f"autotuned in {time.time() - tt:.1f}s: {table['tiers']}")
+    col = torch.arange(kmax, device=dev)[None, :] < k_d[:, None]
+    for name, tiers in (("default_tiers", default), ("autotuned_tiers", table["tiers"])):
+        b = song_amd.Batch(ix, tiers, nq, mode=mode)
+        b.run(q, bids, bfl, pq_arr=pq_d, k_arr=k_d, stats=stats)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(torch.where(col, bids, -1), torch.where(col, pids, -1))) and bool(torch.equal(bfl, pfl))
+        st = stats.cpu().numpy()
+        side.wait_stream(torch.cuda.current_stream())
+        ms, wall = timed_steps(lambda: b.run(q, bids, bfl, pq_arr=pq_d, k_arr=k_d, graph=True, stream=side.cuda_stream),
+                               args.steps, args.warmup, torch, side)
+        results[name] = {"qps": round(nq / (ms / 1e3), 1), "ms_per_step": round(ms, 4),
+                         "wall_ms_per_step": round(wall, 4), "ids_equal_plain": same, "tiers": tiers,
+                         "per_tier_queries": b.counts().tolist(), "algorithmic_bytes": alg_bytes(st, dim, dim * 4, nq)}
+        log(f"{name}: {results[name]['qps']:.0f} q/s, ids equal plain: {same}")
+        b.close()
+
+    # recall per k group on a sample, against the float64 brute force
+    ns = min(nq, args.c4_recall_sample)
+    _, gt_d = groundtruth.knn_fp64(base, query[:ns], kmax, "ip")
+    rec = {}
+    ph = pids.cpu().numpy().astype(np.int64)
+    for kk in sorted(set(k_h.tolist())):
+        sel = np.nonzero(k_h[:ns] == kk)[0]
+        rec[f"recall@{kk}"] = round(groundtruth.recall_at_k(ph[:ns][sel], gt_d[sel], kk, base, query[:ns][sel], "ip"),
+                                    4)
+
+    def cpu_check(p, take, cids):
+        kk = min(p, kmax)
+        keep = np.arange(kk)[None, :] < k_h[take][:, None]
+        return bool(np.array_equal(np.where(keep, cids[:, :kk].astype(np.int64), -1),
+                                   np.where(keep, ph[take][:, :kk], -1)))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(base, graph, "ip", query, groups, args.cpu_seconds, check=cpu_check)
+        cpu["note"] = "the GPU ids compared here are FP32-mode ids; the oracle accumulates in float64"
+    best = results["autotuned_tiers"]
+    ach = best["algorithmic_bytes"] / (best["ms_per_step"] / 1e3)
+    return {
+        "metric": "QPS, batching form: 2M x96 IP, 100k queries with k in {1,10,100} and pq in {32..512}, 1 GPU",
+        "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic: seeded unit-norm Gaussian rows, graph = reference insertion build",
+        "config": {"workload": f"{args.c4_n:,} x {dim} inner product, {nq:,} queries, k uniform in {{1,10,100}}, "
+                               f"pq uniform in {{32,64,128,256,512}}, k capped at pq",
+                   "plain_per_pq_calls": {"qps": round(nq / (ms_plain / 1e3), 1), "ms_per_step": round(ms_plain, 4)},
+                   "batched": results, "recall_sample": ns, **rec},
+        "roofline": {"bound": "hbm", "achieved": round(ach / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK, 5), "traffic": None,
+                     "kernel": "search_persistent<IP, fp32, F32 rows> tiers + bucket/retry (one batched run)"},
+        "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("c2", "c4"), default="c2")
+    ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--pqs", default="256,384,512,640,768,896,1024")
+    ap.add_argument("--pq", type=int, default=0, help="plain form: skip the sweep and time this pq")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="no learned form")
+    ap.add_argument("--no-batch", action="store_true", help="no plain batched form")
+    ap.add_argument("--train-queries", type=int, default=200_000)
+    ap.add_argument("--learned-pqs", default="64,96,128,192,256")
+    ap.add_argument("--step-scales", default="1.5,2.5,0")
+    ap.add_argument("--no-recall", action="store_true",
+                    help="with --pq: skip the ground truth and recall (profiling runs)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run one device copy of this many MiB first (a known-size kernel that "
+                         "calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--c4-n", type=int, default=2_000_000)
+    ap.add_argument("--c4-nq", type=int, default=100_000)
+    ap.add_argument("--c4-recall-sample", type=int, default=10_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries shard across GPUs "
+                                   "with no communication, see DESIGN.md)"}))
+        sys.exit(2)
+    if args.no_recall and not args.pq:
+        ap.error("--no-recall needs --pq")
+
+    import torch
+
+    import song_amd
+
+    out = bench_c4(args, torch, song_amd) if args.workload == "c4" else bench_c2(args, torch, song_amd)
     print(json.dumps(out))
 
 

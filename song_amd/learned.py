@@ -18,8 +18,12 @@ Pipeline (all heavy work on the GPU):
            it and its 3 nearest pivots -- at least the threshold) start at the
            neighbourhood's pivot vertices with budget pq and an expansion cap
            from the stop head; the rest fall back to the plain full-budget
-           search from vertex 0.  The two groups run as two calls of
-           song_search_device_ex with query subsets, on two streams.
+           search from vertex 0.  LearnedSearch.search runs the two groups as
+           two calls of song_search_device_ex with query subsets, on two
+           streams; LearnedBatch turns the split into per-query (pq, entries,
+           cap) arrays on the GPU and runs every query in one batched search
+           (song_batch_run: one tier per budget), so the whole pipeline --
+           features, fused inference, plan, search -- is one hipGraph replay.
 """
 from __future__ import annotations
 
@@ -29,7 +33,7 @@ import time
 
 import numpy as np
 
-from . import SongError, lib, _check
+from . import SongError, lib, _check, Batch, MODE_FP32, tier
 
 
 def _mlp_lib():
@@ -321,3 +325,70 @@ class LearnedSearch:
         if getattr(self, "_side", None) is None:
             self._side = torch.cuda.Stream()
         return self._side
+
+
+class LearnedBatch:
+    """The learned form as one batched search over a fixed query buffer.
+
+    Per query, on the GPU: confident -> (pq_learned, its neighbourhood's pivot vertices,
+    expansion cap from the stop head); fallback -> (pq_fallback, vertex 0 -- entry list
+    padded with -1 --, no cap).  The batch has one tier per budget (tier knobs from the
+    autotuner or defaults).  run() enqueues the pipeline; capture() records it once in a
+    CUDA/HIP graph and replay() relaunches it."""
+
+    def __init__(self, ls, nq, pq_learned, pq_fallback, k=10, conf_threshold=0.9, step_scale=1.5, step_add=8,
+                 storage=0, mode=MODE_FP32, tiers=None, k_ld=None):
+        import torch
+
+        self.ls, self.nq, self.k = ls, nq, k
+        self.pq_l, self.pq_f = int(pq_learned), int(pq_fallback)
+        self.conf_threshold, self.step_scale, self.step_add = conf_threshold, step_scale, step_add
+        dev = ls.dev
+        lo, hi = sorted({self.pq_l, self.pq_f}) if self.pq_l != self.pq_f else (self.pq_l, None)
+        self.tiers = tiers or ([tier(lo)] + ([tier(hi)] if hi else []))
+        self.batch = Batch(ls.index, self.tiers, nq, mode=mode, storage=storage)
+        self.k_ld = k_ld or k
+        self.ids = torch.empty((nq, self.k_ld), dtype=torch.int32, device=dev)
+        self.flags = torch.empty(nq, dtype=torch.int32, device=dev)
+        self.k_arr = torch.full((nq,), k, dtype=torch.int32, device=dev)
+        self.graph = None
+
+    def plan(self, q_dev):
+        """(pq [nq], entries [nq, E] with -1 padding, caps [nq], confident mask) on the GPU."""
+        import torch
+
+        entry, conf, stop = self.ls.predict(q_dev)
+        ok = conf >= self.conf_threshold
+        big = torch.full_like(stop, 1.0e6)
+        caps = torch.where(ok, (self.step_scale * (torch.exp2(stop) - 1.0) + self.step_add).clamp(1, 1.0e6), big)
+        pq = torch.where(ok, torch.full_like(self.k_arr, self.pq_l), torch.full_like(self.k_arr, self.pq_f))
+        entry = torch.where(ok[:, None], entry, torch.full_like(entry, -1))
+        return pq.contiguous(), entry.contiguous(), caps.to(torch.int32).contiguous(), ok
+
+    def run(self, q_dev, stats=None, dists=None):
+        """Enqueue the whole pipeline for q_dev (nq queries); returns the plan."""
+        p = self.plan(q_dev)
+        self.batch.run(q_dev, self.ids, self.flags, k=self.k, pq_arr=p[0], k_arr=self.k_arr, entry=p[1], steps=p[2],
+                       stats=stats, dists=dists)
+        return p
+
+    def capture(self, q_dev):
+        """Record run(q_dev) in a graph (warm-up first); replay() relaunches it on the
+        current contents of q_dev."""
+        import torch
+
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                self.run(q_dev)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._plan_out = self.run(q_dev)
+        torch.cuda.synchronize()
+        return self._plan_out
+
+    def replay(self):
+        self.graph.replay()

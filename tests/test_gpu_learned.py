@@ -154,3 +154,44 @@ def test_learned_entries_beat_vertex_zero(mix50k):
     # the confidence split: every query is served, fallback ones by the plain search
     entry, steps, conf_ids, fb_ids = ls.plan(q, conf_threshold=0.9)
     assert conf_ids.numel() + fb_ids.numel() == q.shape[0]
+
+
+def test_learned_batch_equals_split_search_and_graph_replays(mix50k):
+    """LearnedBatch (one batched search, per-query pq / entries / caps made on the GPU)
+    equals the confident / fallback split run as two plain calls with the same plan; the
+    captured pipeline replays to the same ids with eager work between replays."""
+    import torch
+
+    base, graph, ix, train, test = mix50k
+    ls = learned.LearnedSearch(ix, base, n_pivots=256, hidden=64, seed=1)
+    ls.fit(train, pq_label=64, epochs=6, batch=512, verbose=False)
+    q = torch.from_numpy(test).cuda()
+    nq, k = q.shape[0], 10
+    _, conf, _ = ls.predict(q)
+    thr = float(conf.median().item())  # both groups non-empty
+    lb = learned.LearnedBatch(ls, nq, pq_learned=24, pq_fallback=64, k=k, conf_threshold=thr, step_scale=1.5,
+                              mode=song_amd.MODE_FP32)
+    pq_arr, entry, caps, ok = lb.run(q)
+    torch.cuda.synchronize()
+    got = lb.ids.clone()
+    conf_idx = torch.nonzero(ok).flatten().to(torch.int32)
+    fb_idx = torch.nonzero(~ok).flatten().to(torch.int32)
+    print(f"[learned] batch: {conf_idx.numel()} confident, {fb_idx.numel()} fallback", flush=True)
+    assert conf_idx.numel() > 0 and fb_idx.numel() > 0
+    assert bool((pq_arr[ok] == 24).all()) and bool((pq_arr[~ok] == 64).all())
+    assert bool((entry[~ok] == -1).all()) and bool((entry[ok] >= 0).all())
+    ref = torch.full_like(got, -3)
+    ix.search_device(q, 64, k, ref, subset=fb_idx, mode=song_amd.MODE_FP32, retry=0)
+    ix.search_device(q, 24, k, ref, entry=entry, steps=caps, subset=conf_idx, mode=song_amd.MODE_FP32, retry=0)
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+    assert int(lb.flags.abs().max()) in (0, song_amd.FLAG_STEP_CAP)
+
+    lb.capture(q)
+    for rep in range(3):
+        lb.ids.fill_(-5)
+        x = torch.randn(512, 512, device=q.device)
+        (x @ x).sum().item()
+        lb.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(lb.ids, got), f"replay {rep}"
