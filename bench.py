@@ -136,7 +136,8 @@ def cpu_baseline(base, graph, metric, query, groups, seconds, check=None):
     return {"value": round(done / tcpu, 1), "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"{done} of the {total_q} queries (same share of every pq group), oracle restatement of "
                       f"graph.h:169-216 on {threads} threads, static split; single-thread probe "
-                      f"{1.0 / mean_t:.1f} q/s", "cpu_model": cpu_model(), "ids_equal_gpu": bool(same)}
+                      f"{1.0 / mean_t:.1f} q/s", "cpu_model": cpu_model(),
+            "ids_equal_gpu": bool(same) if check is not None else None}
 
 
 def bench_c2(args, torch, song_amd):
@@ -394,8 +395,9 @@ def bench_c4(args, torch, song_amd):
              torch.empty(len(i), dtype=torch.int32, device=dev)) for p, i, _ in gq]
 
     def plain_all():
+        # retry=1: overflowed queries are re-run with larger tables (complete results, as the batch)
         for (p, _, qq), (o, f) in zip(gq, outs):
-            ix.search_device(qq, p, min(p, kmax), o, None, f, mode=mode, retry=0)
+            ix.search_device(qq, p, min(p, kmax), o, None, f, mode=mode, retry=1)
 
     plain_all()
     torch.cuda.synchronize()
@@ -424,9 +426,12 @@ def bench_c4(args, torch, song_amd):
         side.wait_stream(torch.cuda.current_stream())
         ms, wall = timed_steps(lambda: b.run(q, bids, bfl, pq_arr=pq_d, k_arr=k_d, graph=True, stream=side.cuda_stream),
                                args.steps, args.warmup, torch, side)
+        cnt = b.counts().tolist()
         results[name] = {"qps": round(nq / (ms / 1e3), 1), "ms_per_step": round(ms, 4),
                          "wall_ms_per_step": round(wall, 4), "ids_equal_plain": same, "tiers": tiers,
-                         "per_tier_queries": b.counts().tolist(), "algorithmic_bytes": alg_bytes(st, dim, dim * 4, nq)}
+                         "per_tier_queries": cnt[:-1], "re_runs": cnt[-1],
+                         "mean_distances": round(float(st[:, 1].mean()), 1),
+                         "algorithmic_bytes": alg_bytes(st, dim, dim * 4, nq)}
         log(f"{name}: {results[name]['qps']:.0f} q/s, ids equal plain: {same}")
         b.close()
 
@@ -440,16 +445,11 @@ def bench_c4(args, torch, song_amd):
         rec[f"recall@{kk}"] = round(groundtruth.recall_at_k(ph[:ns][sel], gt_d[sel], kk, base, query[:ns][sel], "ip"),
                                     4)
 
-    def cpu_check(p, take, cids):
-        kk = min(p, kmax)
-        keep = np.arange(kk)[None, :] < k_h[take][:, None]
-        return bool(np.array_equal(np.where(keep, cids[:, :kk].astype(np.int64), -1),
-                                   np.where(keep, ph[take][:, :kk], -1)))
-
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(base, graph, "ip", query, groups, args.cpu_seconds, check=cpu_check)
-        cpu["note"] = "the GPU ids compared here are FP32-mode ids; the oracle accumulates in float64"
+        # FP32-mode GPU ids on float data are not compared with the float64 oracle (the
+        # exact-mode parity is covered by the tests); the sample times the same searches
+        cpu = cpu_baseline(base, graph, "ip", query, groups, args.cpu_seconds)
     best = results["autotuned_tiers"]
     ach = best["algorithmic_bytes"] / (best["ms_per_step"] / 1e3)
     return {

@@ -159,8 +159,8 @@ int song_search_device_ex(song_index *index, const float *d_queries, uint64_t nq
  * hipGraph; song_batch_run_graph captures it once per argument set and replays it.
  * One run: a bucket kernel sorts the queries into tiers by their pq, one persistent
  * kernel per tier searches its list (waves take queries from a device work counter,
- * LDS sized for the tier), and one persistent kernel re-runs queries that overflowed
- * with the retry configuration.  Each query's result equals song_search_device's at the
+ * LDS sized for the tier; tiers on forked streams joined by events), and one
+ * persistent kernel re-runs queries that overflowed with the retry configuration.  Each query's result equals song_search_device's at the
  * same (k, pq), entries, step cap, mode and storage.
  */
 #define SONG_MAX_TIERS 8
@@ -183,15 +183,20 @@ typedef struct {
     int storage;   /* SONG_STORE_* */
     int rerank;    /* low-precision storage: exact FP32 re-rank of the pq list */
     int max_steps; /* expansion cap of every query; 0 = 1,000,000 */
-    int spill_log2; /* global visited spill table per wave: 2^spill_log2 u32 (10..18) */
+    int spill_log2; /* global visited spill table per wave: 2^spill_log2 u32 (10..18, default 14;
+                       the re-run's tables are 4x larger) */
     int n_tiers;   /* 1..SONG_MAX_TIERS, ascending pq_max */
     song_tier tier[SONG_MAX_TIERS];
     song_tier retry; /* re-run of overflowed queries (pq_max ignored: the largest tier's) */
+    int parallel_tiers; /* 1: tiers run on forked streams, largest pq first, so one tier's
+                           workgroups fill the slots another's finished waves free; 0: in
+                           order on the run's stream */
 } song_batch_config;
 
 typedef struct song_batch song_batch;
 
-/* One tier up to pq_max with default knobs; retry with 4x the filter and heap depth. */
+/* One tier up to pq_max with default knobs; the re-run with 4x the filter, heap depth and
+ * spill table, on a full-occupancy grid. */
 void song_batch_config_default(song_batch_config *cfg, int pq_max);
 
 /* The mode SONG_MODE_AUTO would pick for these device queries (synchronizes stream). */

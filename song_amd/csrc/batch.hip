@@ -1,15 +1,19 @@
 // Batching form of the search (include/song_amd.h, song_batch_*): queries with uneven
 // k and pq in one run, persistent per-tier kernels, capture-safe enqueue, hipGraph replay.
 //
-// One run enqueues, on one stream:
+// One run enqueues, on the caller's stream:
 //   1. batch_reset_kernel   zero the run's counters (tier sizes, work counters, retry count)
 //   2. batch_bucket_kernel  query i -> the first tier with pq_i <= pq_max, appended to that
 //                           tier's list (wave-aggregated atomics); no tier -> BAD_PARAMS
-//   3. search_persistent    one launch per tier: its grid fills the GPU at the tier's LDS
-//                           size, each wave takes list positions from the tier's counter
-//   4. search_persistent    the retry configuration over the queries step 3 flagged
-// Nothing here allocates, clears memory with a memset or waits on the host, so the four
-// steps can be captured into a hipGraph as they are.
+//   3. search_persistent    one launch per tier, largest pq first: its grid fills the GPU at
+//                           the tier's LDS size, each wave takes list positions from the
+//                           tier's counter.  With parallel tiers the launches after the first
+//                           go to forked streams (event fork / join), so a tier's workgroups
+//                           take the slots that another tier's finished waves free
+//   4. search_persistent    the retry configuration over the queries step 3 flagged, after
+//                           the join
+// Nothing here allocates, clears memory with a memset or waits on the host, so the steps
+// can be captured into a hipGraph as they are (the fork / join becomes graph edges).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,6 +65,7 @@ struct Launch {
     int groups = 0, waves = 1;
     size_t wave_lds = 0;
     uint32_t *gvis = nullptr;
+    int ghlog = 13;  // spill table per wave: 2^ghlog u32
     int pq_max = 0;
 };
 
@@ -92,6 +97,9 @@ struct song_batch {
     uint32_t *d_lists = nullptr;  // T x nq_max
     uint32_t *d_retry = nullptr;  // nq_max
     std::vector<uint32_t *> allocs;
+    // parallel tiers: one stream per tier after the first and the fork / join events
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;
     struct Cached {
         RunArgs a;
         void *stream;
@@ -112,7 +120,7 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     c->storage = SONG_STORE_F32;
     c->rerank = 1;
     c->max_steps = 0;
-    c->spill_log2 = 13;
+    c->spill_log2 = 14;
     c->n_tiers = 1;
     song_tier t;
     t.pq_max = pq_max;
@@ -125,7 +133,8 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     c->retry = t;
     c->retry.hash_log2 = 0;  // the largest tier's filter x 4
     c->retry.slack = 4 * 32 + 64;
-    c->retry.groups_per_cu = 1;
+    c->retry.groups_per_cu = 0;
+    c->parallel_tiers = 1;
 }
 
 extern "C" int song_resolve_mode(const song_index *ix, const float *d_queries, uint64_t nq, int *mode, void *stream) {
@@ -157,6 +166,8 @@ static int check_tier(const song_tier &t, int idx) {
 extern "C" int song_batch_destroy(song_batch *b) {
     if (!b) return SONG_OK;
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+    for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+    for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
     for (uint32_t *p : b->allocs) (void)hipFree(p);
     delete b;
     return SONG_OK;
@@ -172,7 +183,7 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         return fail(SONG_E_ARG, "batch mode must be EXACT or FP32 (song_resolve_mode picks one)");
     if (c.storage < SONG_STORE_F32 || c.storage > SONG_STORE_FP8) return fail(SONG_E_ARG, "storage=%d", c.storage);
     if (c.n_tiers < 1 || c.n_tiers > SONG_MAX_TIERS) return fail(SONG_E_ARG, "n_tiers=%d", c.n_tiers);
-    const int spill = c.spill_log2 ? c.spill_log2 : 13;
+    const int spill = c.spill_log2 ? c.spill_log2 : 14;
     if (spill < 10 || spill > 18) return fail(SONG_E_ARG, "spill_log2=%d not in [10,18]", c.spill_log2);
     for (int t = 0; t < c.n_tiers; ++t) {
         int rc = check_tier(c.tier[t], t);
@@ -233,7 +244,9 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         }
         L.groups = per_cu * b->n_cu;
         const size_t slots = (size_t)L.groups * L.waves;
-        if (hipMalloc(&L.gvis, slots * ((size_t)4 << spill)) != hipSuccess)
+        // the re-run gets spill tables 4x larger than the tiers'
+        L.ghlog = is_retry ? std::min(18, spill + 2) : spill;
+        if (hipMalloc(&L.gvis, slots * ((size_t)4 << L.ghlog)) != hipSuccess)
             return cleanup(fail(SONG_E_HIP, "tier %d: spill tables (%zu slots)", t, slots));
         b->allocs.push_back(L.gvis);
         b->tiers.push_back(L);
@@ -244,6 +257,21 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         (b->allocs.push_back(b->d_lists), hipMalloc(&b->d_retry, sizeof(uint32_t) * nq_max)) != hipSuccess)
         return cleanup(fail(SONG_E_HIP, "batch work lists"));
     b->allocs.push_back(b->d_retry);
+    if (c.parallel_tiers && T > 1) {
+        // default-priority, non-blocking streams; event 0 forks, events 1.. join
+        for (int t = 1; t < T; ++t) {
+            hipStream_t s = nullptr;
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+                return cleanup(fail(SONG_E_HIP, "tier stream"));
+            b->side.push_back(s);
+        }
+        for (int t = 0; t < T; ++t) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return cleanup(fail(SONG_E_HIP, "tier event"));
+            b->ev.push_back(e);
+        }
+    }
     if (hipMemset(b->d_ctr, 0, sizeof(uint32_t) * (2 * T + 2)) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
         return cleanup(fail(SONG_E_HIP, "batch counters"));
@@ -305,9 +333,21 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
     sp.rerank = c.rerank;
     sp.ghlog = c.spill_log2;
     sp.qbase = 0;
-    for (int t = 0; t <= T; ++t) {
+    // launch order: tiers by descending pq (longest searches first), then the retry; with
+    // parallel tiers every tier but the first launched runs on its own forked stream
+    const bool par = !b->side.empty();
+    if (par) HIP_TRY(hipEventRecord(b->ev[0], st));
+    for (int o = 0; o <= T; ++o) {
+        const int t = o < T ? T - 1 - o : T;
         const Launch &L = b->tiers[t];
         const bool is_retry = t == T;
+        hipStream_t ls = st;
+        if (par && o > 0 && o < T) {
+            ls = b->side[o - 1];
+            HIP_TRY(hipStreamWaitEvent(ls, b->ev[0], 0));
+        }
+        if (par && is_retry)
+            for (int j = 1; j < T; ++j) HIP_TRY(hipStreamWaitEvent(st, b->ev[j], 0));
         SearchParams p = sp;
         // without per-query budgets every query has the call's pq (the LDS slice is sized
         // for the tier's pq_max >= pq)
@@ -320,6 +360,7 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
         p.scale = ix->d_scale[L.plan.store];
         p.lp_stride = ix->lp_stride[L.plan.store];
         p.gvis = L.gvis;
+        p.ghlog = L.ghlog;
         p.wave_lds = (uint32_t)L.wave_lds;
         if (!is_retry) {
             p.qidx = b->d_lists + (size_t)t * b->nq_max;
@@ -334,8 +375,9 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
             p.retry_list = nullptr;
             p.retry_count = nullptr;
         }
-        hipLaunchKernelGGL(L.fn, dim3((unsigned)L.groups), dim3(64 * L.waves), L.wave_lds * L.waves, st, p);
+        hipLaunchKernelGGL(L.fn, dim3((unsigned)L.groups), dim3(64 * L.waves), L.wave_lds * L.waves, ls, p);
         HIP_TRY(hipGetLastError());
+        if (ls != st) HIP_TRY(hipEventRecord(b->ev[o], ls));
     }
     return SONG_OK;
 }

@@ -80,7 +80,14 @@ def _batch_run(b, q, k_d, pq_d, k_ld=100, graph=False, entry=None, steps=None):
     ids = torch.full((nq, k_ld), -7, dtype=torch.int32, device=q.device)
     ds = torch.zeros((nq, k_ld), dtype=torch.float32, device=q.device)
     fl = torch.full((nq,), -1, dtype=torch.int32, device=q.device)
-    b.run(q, ids, fl, pq_arr=pq_d, k_arr=k_d, dists=ds, graph=graph, entry=entry, steps=steps)
+    if graph:  # captured on a side stream (graph runs need a stream other than the default)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        for _ in range(2):  # capture, then one replay
+            b.run(q, ids, fl, pq_arr=pq_d, k_arr=k_d, dists=ds, graph=True, entry=entry, steps=steps,
+                  stream=side.cuda_stream)
+    else:
+        b.run(q, ids, fl, pq_arr=pq_d, k_arr=k_d, dists=ds, graph=graph, entry=entry, steps=steps)
     torch.cuda.synchronize()
     return ids, ds, fl
 
@@ -121,6 +128,15 @@ def test_batch_equals_plain_per_query(c1, name):
     ids = got[0][:ng].cpu().numpy().astype(np.int64)
     assert np.array_equal(ids, c1["ref_ids"][:, :100])
     del torch
+
+
+def test_serial_and_parallel_tiers_agree(c1):
+    want = _plain(c1["ix"], c1["q"], c1["k"], c1["pq"], song_amd.MODE_EXACT)
+    for par in (False, True):
+        b = song_amd.Batch(c1["ix"], [tier(64), tier(128, waves_per_group=2), tier(512)], c1["q"].shape[0],
+                           mode=song_amd.MODE_EXACT, parallel_tiers=par)
+        _same(_batch_run(b, c1["q"], c1["k_d"], c1["pq_d"]), want, f"parallel={par}")
+        _same(_batch_run(b, c1["q"], c1["k_d"], c1["pq_d"], graph=True), want, f"graph parallel={par}")
 
 
 def test_batch_overflow_reruns_on_ties():
