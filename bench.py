@@ -218,6 +218,11 @@ def bench_c2(args, torch, song_amd):
                   "mean_distances": round(float(st[:, 1].mean()), 1), "arithmetic": "fp32 (exact on integer data)"}
     log(f"plain form: {qps_plain:.0f} q/s at pq={pq} recall={r:.4f}")
     alg_plain = alg_bytes(st, dim, dim * 4, nq)
+    # host buffers in and out (song_search: device buffers allocated per call, queries up,
+    # ids and distances down over PCIe) -- reported beside the resident-input rate
+    ms_host = timed(lambda: ix.search(query, pq, k, mode=mode), 3, torch)
+    plain_info["host_buffers_qps"] = round(nq / (ms_host / 1e3), 1)
+    log(f"plain form with host buffers: {plain_info['host_buffers_qps']:.0f} q/s")
 
     # ---------------------------------------------------------------- plain, batched
     batched_info = None
