@@ -13,15 +13,16 @@
 // expansion order -- and therefore the output -- equals the CPU's.
 //
 // Per-wave LDS layout (bytes, in order):
-//   visited  : H x u32 open-addressing set of (id + 1), linear probing, no deletion
+//   visited  : H x u32 open-addressing set of (id + 1), probed in groups of 4, no deletion
 //   queue    : qcap = pq + slack + 128 keys, sorted ascending, live range [h, h + nq), h < 64
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
 //   cid      : 64 u32    candidate ids of the current expansion
 //   qv       : qld f32   the query, zero-padded to a multiple of 16
 //
-// One wave per workgroup: LDS instructions of a wave execute in issue order, so lanes
-// exchange data through LDS with only a compiler fence between the write and the read
-// (lds_fence), and global loads stay in flight across those points.
+// Each wave owns its LDS slice (one wave per workgroup in search_kernel; 1-4 waves, each
+// on its own slice, in search_persistent).  LDS instructions of a wave execute in issue
+// order, so lanes exchange data through LDS with only a compiler fence between the write
+// and the read (lds_fence), and global loads stay in flight across those points.
 //
 // Per expansion the dependent global round trips are: adjacency row of the expanded
 // vertex (prefetched one step ahead for the predicted next vertex, the queue head after
@@ -161,28 +162,40 @@ __device__ __forceinline__ int lsb_exp(float p) {
     return e - 150 + __builtin_ctz(m);
 }
 
-// visited set: insert id, return true if it was absent.  Lanes inserting the same id
-// race through compare-and-swap; exactly one of them sees the slot empty.
+// visited set: (id + 1) in an open-addressing table probed in aligned groups of 4 slots,
+// one 16-byte LDS read per group.  An id goes to the first empty slot of the first group
+// on its probe sequence that has one; with no deletions a lookup can therefore stop at the
+// first group holding an empty slot.  Lanes inserting the same id race through
+// compare-and-swap; exactly one of them sees the slot empty.  Returns true if id was absent.
 __device__ __forceinline__ bool vis_insert(uint32_t *vis, uint32_t mask, int hlog, uint32_t id) {
-    uint32_t key = id + 1u;
-    uint32_t s = (id * 0x9E3779B1u) >> (32 - hlog);
+    const uint32_t key = id + 1u;
+    const uint32_t gmask = mask >> 2;
+    uint32_t g = (id * 0x9E3779B1u) >> (34 - hlog);
     for (;;) {
-        uint32_t old = atomicCAS(&vis[s], 0u, key);
+        const uint4 v = *reinterpret_cast<const uint4 *>(vis + (g << 2));
+        if (v.x == key || v.y == key || v.z == key || v.w == key) return false;
+        const uint32_t e = v.x == 0u ? 0u : v.y == 0u ? 1u : v.z == 0u ? 2u : v.w == 0u ? 3u : 4u;
+        if (e == 4u) {
+            g = (g + 1u) & gmask;
+            continue;
+        }
+        const uint32_t old = atomicCAS(&vis[(g << 2) + e], 0u, key);
         if (old == 0u) return true;
         if (old == key) return false;
-        s = (s + 1u) & mask;
+        lds_fence();  // another id took the slot: read the group again
     }
 }
 
-// read-only probe of the LDS table
+// read-only lookup in the LDS table
 __device__ __forceinline__ bool vis_test(const uint32_t *vis, uint32_t mask, int hlog, uint32_t id) {
     const uint32_t key = id + 1u;
-    uint32_t s = (id * 0x9E3779B1u) >> (32 - hlog);
+    const uint32_t gmask = mask >> 2;
+    uint32_t g = (id * 0x9E3779B1u) >> (34 - hlog);
     for (;;) {
-        const uint32_t v = vis[s];
-        if (v == key) return true;
-        if (v == 0u) return false;
-        s = (s + 1u) & mask;
+        const uint4 v = *reinterpret_cast<const uint4 *>(vis + (g << 2));
+        if (v.x == key || v.y == key || v.z == key || v.w == key) return true;
+        if (v.x == 0u || v.y == 0u || v.z == 0u || v.w == 0u) return false;
+        g = (g + 1u) & gmask;
     }
 }
 
@@ -431,6 +444,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                                              unsigned char *smem) {
     using Key = typename KeySel<EXACT>::T;
     constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
+    constexpr int MB = 4;  // queue blocks moved per batch of LDS reads
     const int lane = lane_id();
     lds_fence();
     // per-query budgets are clamped to the launch's LDS sizing (pq) and output rows (out_ld)
@@ -447,7 +461,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));
     float *qv = reinterpret_cast<float *>(cid + 64);
 
-    for (uint32_t i = lane; i < H; i += 64) vis[i] = 0u;
+    for (uint32_t i = lane; i < (H >> 2); i += 64) reinterpret_cast<uint4 *>(vis)[i] = make_uint4(0u, 0u, 0u, 0u);
     for (int i = lane; i < P.qld; i += 64) qv[i] = (i < P.dim) ? P.queries[(size_t)qi * P.dim + i] : 0.0f;
     lds_fence();
 
@@ -532,20 +546,35 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195):
         // find its position, shift the entries behind it one place (last block first)
         {
-            const int p = prefix_count(B, nbest, [&](const Key &e) { return lt(e, now); });
+            // the list's last block [lb, nbest): `now` usually lands there (expansions come
+            // out in nearly ascending order), so one read gives the position and the entries
+            // to shift; otherwise search [0, lb)
+            const int lb = nbest > 64 ? nbest - 64 : 0;
+            const int i0 = lb + lane;
+            const bool v0 = i0 < nbest;
+            const Key e0 = v0 ? B[i0] : Key::inf();
+            const uint64_t below0 = ballot(v0 && lt(e0, now));
+            const bool in_last = lb == 0 || (below0 & 1ull);
+            const int p = in_last ? lb + __popcll(below0)
+                                  : prefix_count(B, lb, [&](const Key &e) { return lt(e, now); });
             if (p < k) settle = steps;
             if (p < pq) {
                 const int last = min(nbest, pq - 1);  // entries [p, last) move to [p+1, last+1)
-                for (int blk = (last - 1) >> 6; blk >= (p >> 6) && last > p; --blk) {
-                    const int i = blk * 64 + lane;
-                    const bool live = i >= p && i < last;
-                    const Key e = B[live ? i : 0];
-                    if (live) B[i + 1] = e;
+                if (in_last) {
+                    if (v0 && i0 >= p && i0 < last) B[i0 + 1] = e0;
+                } else {
+                    for (int blk = (last - 1) >> 6; blk >= (p >> 6) && last > p; --blk) {
+                        const int i = blk * 64 + lane;
+                        const bool live = i >= p && i < last;
+                        const Key e = B[live ? i : 0];
+                        if (live) B[i + 1] = e;
+                    }
                 }
                 if (lane == 0) B[p] = now;
+                // the new largest entry: `now` itself, or the one that was at pq - 2 (still in e0)
+                if (nbest + 1 >= pq) bmax = (p == pq - 1) ? now.dk() : e0.rl(pq - 2 - lb).dk();
                 nbest = min(nbest + 1, pq);
                 lds_fence();
-                if (nbest == pq) bmax = B[pq - 1].dk();
             }
         }
 
@@ -593,11 +622,16 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (ms == 0) continue;
 
         if (h >= 64) {
-            // move the live range back to the buffer start (left moves, first block first)
-            for (int blk = 0; blk * 64 < nq; ++blk) {
-                const int i = blk * 64 + lane;
-                const Key e = Q[h + min(i, nq - 1)];
-                if (i < nq) Q[i] = e;
+            // move the live range back to the buffer start (left moves, first blocks first;
+            // MB blocks are read before any of them is written: sources lie at or beyond the
+            // targets of the same and earlier batches)
+            for (int blk = 0; blk * 64 < nq; blk += MB) {
+                Key e[MB];
+#pragma unroll
+                for (int c = 0; c < MB; ++c) e[c] = Q[h + min((blk + c) * 64 + lane, nq - 1)];
+#pragma unroll
+                for (int c = 0; c < MB; ++c)
+                    if ((blk + c) * 64 + lane < nq) Q[(blk + c) * 64 + lane] = e[c];
             }
             h = 0;
             lds_fence();
@@ -609,23 +643,39 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             if (lt(o, smin)) smin = o;
             rank += lt(o, mine) ? 1 : 0;
         }
-        // entries [0, p0) stay; entries [p0, nq) move right by the survivors below them
+        // entries [0, p0) stay; entries [p0, nq) move right by the survivors below them.
+        // Blocks are processed last first, MB at a time: the batch's reads are all issued
+        // before its writes, and a right shift only writes at or beyond the batch's own
+        // blocks, which later (lower) batches never read.
         const int p0 = prefix_count(Q + h, nq, [&](const Key &e) { return lt(e, smin); });
         int acc = 0;  // this lane's survivor: queue entries in [p0, nq) below it
-        for (int blk = (nq - 1) >> 6; nq > p0 && blk >= (p0 >> 6); --blk) {
-            const int i = blk * 64 + lane;
-            const bool live = i >= p0 && i < nq;
-            const Key e = live ? Q[h + i] : Key::inf();
-            int cnt = 0;
+        const int bfirst = p0 >> 6;
+        for (int top = (nq - 1) >> 6; nq > p0 && top >= bfirst; top -= MB) {
+            Key e[MB];
+            bool live[MB];
+            int cnt[MB];
+#pragma unroll
+            for (int c = 0; c < MB; ++c) {
+                const int i = (top - c) * 64 + lane;
+                live[c] = i >= p0 && i < nq;
+                e[c] = live[c] ? Q[h + i] : Key::inf();
+                cnt[c] = 0;
+            }
             for (uint64_t bits = sm; bits; bits &= bits - 1) {
                 const int j = __builtin_ctzll(bits);
                 const Key o = mine.rl(j);
-                const bool sb = lt(o, e);
-                cnt += sb ? 1 : 0;
-                const int below = __popcll(ballot(live && !sb));
+                int below = 0;
+#pragma unroll
+                for (int c = 0; c < MB; ++c) {
+                    const bool sb = lt(o, e[c]);
+                    cnt[c] += sb ? 1 : 0;
+                    below += __popcll(ballot(live[c] && !sb));
+                }
                 if (lane == j) acc += below;
             }
-            if (live) Q[h + i + cnt] = e;
+#pragma unroll
+            for (int c = 0; c < MB; ++c)
+                if (live[c]) Q[h + (top - c) * 64 + lane + cnt[c]] = e[c];
         }
         if (surv) Q[h + p0 + acc + rank] = mine;
         nq += ms;
