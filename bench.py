@@ -24,7 +24,9 @@ Forms timed:
                  replay per step.  `value` is its QPS at the fastest setting whose
                  recall reaches the target.
 `--workload c4` measures the batching form on its own workload instead (2M x 96
-inner product, 100,000 queries with uneven k and pq).  Prints one JSON line.
+inner product, 100,000 queries with uneven k and pq); `--workload c3` the precision
+form (factor-model 960-d L2: recall deltas and distance errors of BF16 / FP8 rows).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -255,7 +257,7 @@ def bench_c2(args, torch, song_amd):
 
         ix.add_storage(song_amd.STORE_BF16)
         train = datasets.mixture_l2_queries(args.train_queries, seed=2, stream=1)
-        ls = learned.LearnedSearch(ix, base, n_pivots=4096, hidden=256, seed=0)
+        ls = learned.LearnedSearch(ix, base, n_pivots=4096, hidden=256, seed=0, n_nbr=8)
         t2 = time.time()
         fit = ls.fit(train, pq_label=256, k=k, epochs=8, batch=1024, lr=2e-3, verbose=False)
         log(f"learned predictor trained in {time.time() - t2:.1f}s: top1 acc {fit['train_top1_acc']:.3f} "
@@ -473,12 +475,76 @@ def bench_c4(args, torch, song_amd):
     }
 
 
+def bench_c3(args, torch, song_amd):
+    """Precision form on its workload: rank-64 Gaussian factor model + N(0, 0.05) noise, min-max
+    scaled to [0,1], dim 960, L2, 1,000 queries.  Rows searched as FP32 (float64 accumulation =
+    the exact form, and float32 accumulation), BF16 and FP8-E4M3 with per-row scales (FP32
+    accumulation; exact FP32 re-rank of the final pq list, and without it to expose the raw
+    low-precision distance error).  Reports recall@10 / recall@100 against the float64 brute force,
+    the recall deltas against FP32 and FP64, the distance errors and the QPS of each."""
+    from song_amd import datasets, groundtruth, workloads
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    base, query = datasets.factor_l2(n_base=args.c3_n, n_query=args.c3_nq)
+    graph, bst = workloads.graph_for(base, "l2", verbose=True)
+    log(f"c3 workload ready in {time.time() - t0:.1f}s (graph cached={bst.get('cached')})")
+    ix = song_amd.Index(base, graph, "l2")
+    ix.add_storage(song_amd.STORE_BF16)
+    ix.add_storage(song_amd.STORE_FP8)
+    nq, dim = query.shape
+    _, gt_d = groundtruth.knn_fp64(base, query, 100, "l2")
+    q = torch.from_numpy(query).to(dev)
+    pq = args.c3_pq
+    ids = torch.empty((nq, 100), dtype=torch.int32, device=dev)
+    ds = torch.empty((nq, 100), dtype=torch.float32, device=dev)
+    fl = torch.empty(nq, dtype=torch.int32, device=dev)
+    forms = [("fp64_accum", song_amd.MODE_EXACT, song_amd.STORE_F32, 1),
+             ("fp32", song_amd.MODE_FP32, song_amd.STORE_F32, 1),
+             ("bf16_rerank", song_amd.MODE_FP32, song_amd.STORE_BF16, 1),
+             ("fp8_rerank", song_amd.MODE_FP32, song_amd.STORE_FP8, 1),
+             ("bf16_raw", song_amd.MODE_FP32, song_amd.STORE_BF16, 0),
+             ("fp8_raw", song_amd.MODE_FP32, song_amd.STORE_FP8, 0)]
+    res = {}
+    for name, mode, store, rerank in forms:
+        def run():
+            ix.search_device(q, pq, 100, ids, ds, fl, mode=mode, storage=store, rerank=rerank)
+        run()
+        torch.cuda.synchronize()
+        i_h = ids.cpu().numpy().astype(np.int64)
+        d_h = ds.cpu().numpy().astype(np.float64)
+        exact = groundtruth.exact_distances(base, query, i_h, "l2")
+        ok = i_h >= 0
+        rel = np.abs(d_h - exact)[ok] / np.maximum(np.abs(exact[ok]), 1e-30)
+        ms = timed(run, 5, torch)
+        res[name] = {"recall@10": round(groundtruth.recall_at_k(i_h, gt_d, 10, base, query, "l2"), 4),
+                     "recall@100": round(groundtruth.recall_at_k(i_h, gt_d, 100, base, query, "l2"), 4),
+                     "max_rel_dist_err_vs_fp64": float(f"{rel.max():.3e}"),
+                     "mean_rel_dist_err_vs_fp64": float(f"{rel.mean():.3e}"),
+                     "qps": round(nq / (ms / 1e3), 1), "ms": round(ms, 4)}
+        log(f"c3 {name}: {res[name]}")
+    for name in ("bf16_rerank", "fp8_rerank", "bf16_raw", "fp8_raw"):
+        for ref in ("fp32", "fp64_accum"):
+            for kk in (10, 100):
+                res[name][f"recall@{kk}_delta_vs_{ref}"] = round(res[name][f"recall@{kk}"] - res[ref][f"recall@{kk}"], 4)
+    best = res["bf16_rerank"]
+    return {
+        "metric": "precision form: recall delta and distance error, factor-model 960-d L2, BF16 / FP8-E4M3 rows",
+        "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": best["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic: seeded rank-64 Gaussian factor model, graph = reference insertion build",
+        "config": {"workload": f"{args.c3_n:,} x {dim} L2 in [0,1], {nq:,} queries, pq {pq}, k 10 and 100",
+                   "forms": res},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("c2", "c4"), default="c2")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--pqs", default="256,384,512,640,768,896,1024")
@@ -498,6 +564,9 @@ def main():
     ap.add_argument("--c4-n", type=int, default=2_000_000)
     ap.add_argument("--c4-nq", type=int, default=100_000)
     ap.add_argument("--c4-recall-sample", type=int, default=10_000)
+    ap.add_argument("--c3-n", type=int, default=1_000_000)
+    ap.add_argument("--c3-nq", type=int, default=1000)
+    ap.add_argument("--c3-pq", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries shard across GPUs "
@@ -510,7 +579,7 @@ def main():
 
     import song_amd
 
-    out = bench_c4(args, torch, song_amd) if args.workload == "c4" else bench_c2(args, torch, song_amd)
+    out = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args, torch, song_amd)
     print(json.dumps(out))
 
 

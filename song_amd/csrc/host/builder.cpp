@@ -19,6 +19,8 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -338,15 +340,35 @@ extern "C" int song_graph_build(const float *data, uint64_t n, int dim, int metr
     std::vector<SearchCtx> ctx(threads);
     for (auto &c : ctx) c.init(n);
     std::vector<int64_t> mod(n, -1);
-    const uint64_t W = threads == 1 ? 1 : (uint64_t)threads * 8;
-    std::vector<std::vector<uint32_t>> reads(W);
-    std::vector<uint32_t> res(W * kBudget);
-    std::vector<int> got(W);
+    // Window size adapts to the conflict rate: a search is redone (sequentially) when an
+    // earlier commit of its window changed a row it read, so a window holds between
+    // `threads` and 8 x `threads` searches: halved when over a quarter of the last window
+    // was redone, doubled when under a twentieth was.
+    const uint64_t Wmin = threads == 1 ? 1 : (uint64_t)threads, Wmax = threads == 1 ? 1 : (uint64_t)threads * 8;
+    uint64_t W = threads == 1 ? 1 : (uint64_t)threads * 2;
+    std::vector<std::vector<uint32_t>> reads(Wmax);
+    std::vector<uint32_t> res(Wmax * kBudget);
+    std::vector<int> got(Wmax);
     uint64_t searches = 0, redone = 0;
     Pool pool(threads);
 
-    for (uint64_t s = 0; s < n; s += W) {
-        const uint64_t e = std::min(n, s + W);
+    // SONG_BUILD_PROGRESS set: a line on stderr about every 20 s of building
+    const bool progress = std::getenv("SONG_BUILD_PROGRESS") != nullptr;
+    auto last_report = t0;
+    for (uint64_t s = 0, e = 0; s < n; s = e) {
+        e = std::min(n, s + W);
+        const uint64_t redone0 = redone;
+        if (progress) {
+            const auto now = std::chrono::steady_clock::now();
+            if (std::chrono::duration<double>(now - last_report).count() >= 20.0) {
+                last_report = now;
+                std::fprintf(stderr, "[build] %llu / %llu vertices, %.0f s, %llu redone, window %llu\n",
+                             (unsigned long long)s, (unsigned long long)n,
+                             std::chrono::duration<double>(now - t0).count(), (unsigned long long)redone,
+                             (unsigned long long)W);
+                std::fflush(stderr);
+            }
+        }
         if (threads > 1 && e - s > 1) {
             pool.run(e - s, [&](int t, uint64_t i) {
                 uint32_t v = (uint32_t)(s + i);
@@ -368,6 +390,11 @@ extern "C" int song_graph_build(const float *data, uint64_t n, int dim, int metr
                 if (threads > 1 && e - s > 1) ++redone;
             }
             G.commit((uint32_t)v, &res[i * kBudget], got[i], mod);
+        }
+        if (threads > 1) {
+            const uint64_t r = redone - redone0, w = e - s;
+            if (r * 4 > w) W = std::max(Wmin, W / 2);
+            else if (r * 20 < w) W = std::min(Wmax, W * 2);
         }
     }
     if (stats) {

@@ -45,6 +45,8 @@ def graph_for(base: np.ndarray, metric: str, threads: int = 0, verbose: bool = F
     if os.path.exists(path) and os.path.getsize(path) == n * 32 * 8:
         return np.fromfile(path, dtype=np.uint64).reshape(n, 32), {"cached": True}
     t = time.time()
+    if verbose:
+        os.environ["SONG_BUILD_PROGRESS"] = "1"  # the builder reports on stderr every ~20 s
     g, st = build_graph(base, metric, threads=threads, return_stats=True)
     st["cached"] = False
     if verbose:
