@@ -191,6 +191,8 @@ typedef struct {
     int parallel_tiers; /* 1: tiers run on forked streams, largest pq first, so one tier's
                            workgroups fill the slots another's finished waves free; 0: in
                            order on the run's stream */
+    int priority_first; /* 1: with parallel tiers, the largest-pq tier's waves run at raised
+                           issue priority (its searches are the longest) */
 } song_batch_config;
 
 typedef struct song_batch song_batch;

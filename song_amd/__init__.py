@@ -50,7 +50,8 @@ MAX_TIERS = 8
 class BatchConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("storage", ctypes.c_int), ("rerank", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("spill_log2", ctypes.c_int), ("n_tiers", ctypes.c_int),
-                ("tier", Tier * MAX_TIERS), ("retry", Tier), ("parallel_tiers", ctypes.c_int)]
+                ("tier", Tier * MAX_TIERS), ("retry", Tier), ("parallel_tiers", ctypes.c_int),
+                ("priority_first", ctypes.c_int)]
 
 
 def lib():
@@ -249,7 +250,7 @@ class Batch:
     (None: 4x the largest tier's filter and heap depth)."""
 
     def __init__(self, index, tiers, nq_max, mode=MODE_EXACT, storage=STORE_F32, rerank=1, max_steps=0,
-                 spill_log2=14, retry=None, parallel_tiers=True):
+                 spill_log2=14, retry=None, parallel_tiers=True, priority_first=True):
         if not 1 <= len(tiers) <= MAX_TIERS:
             raise SongError(f"{len(tiers)} tiers (1..{MAX_TIERS})")
         cfg = BatchConfig()
@@ -257,6 +258,7 @@ class Batch:
         cfg.mode, cfg.storage, cfg.rerank, cfg.max_steps, cfg.spill_log2 = mode, storage, rerank, max_steps, spill_log2
         cfg.n_tiers = len(tiers)
         cfg.parallel_tiers = 1 if parallel_tiers else 0
+        cfg.priority_first = 1 if priority_first else 0
         for i, t in enumerate(tiers):
             for key in TIER_KEYS:
                 setattr(cfg.tier[i], key, int(t.get(key, getattr(cfg.tier[0], key) if key != "pq_max" else 0)))

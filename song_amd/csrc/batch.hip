@@ -135,6 +135,7 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     c->retry.slack = 4 * 32 + 64;
     c->retry.groups_per_cu = 0;
     c->parallel_tiers = 1;
+    c->priority_first = 1;
 }
 
 extern "C" int song_resolve_mode(const song_index *ix, const float *d_queries, uint64_t nq, int *mode, void *stream) {
@@ -362,6 +363,9 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
         p.gvis = L.gvis;
         p.ghlog = L.ghlog;
         p.wave_lds = (uint32_t)L.wave_lds;
+        // with tiers running side by side, the largest-pq tier (launched first, the longest
+        // searches) gets issue priority
+        p.wave_prio = (par && o == 0 && c.priority_first) ? 1 : 0;
         if (!is_retry) {
             p.qidx = b->d_lists + (size_t)t * b->nq_max;
             p.work_count = count + t;

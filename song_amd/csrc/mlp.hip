@@ -277,32 +277,43 @@ struct InferArgs {
 };
 
 // One dense layer on the LDS tile: accumulators of in[64 rows] . W[n]^T for the 64-column
-// slice [n0, n0+64) of this wave (4x4 MFMA tiles).
+// slice [n0, n0+64) of this wave (4x4 MFMA tiles).  K is a multiple of 16, so every
+// fragment (4 consecutive k of one row or column) is one aligned 8-byte load; the next
+// k-step's weight fragments are loaded while the current step's MFMAs run.
 __device__ __forceinline__ void tile_layer(const uint16_t *in, int ldin, int K, const uint16_t *W, int N, int n0,
                                            v4f (&acc)[4][4]) {
     const int lane = threadIdx.x & 63;
+    const int kq = 4 * (lane >> 4);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *wrow[4];
+    bool ok[4];
+    v4s bf[4];
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+        const int n = n0 + ni * 16 + (lane & 15);
+        ok[ni] = n < N;
+        wrow[ni] = W + (long long)(ok[ni] ? n : 0) * K + kq;
+        bf[ni] = ok[ni] ? *reinterpret_cast<const v4s *>(wrow[ni]) : v4s{0, 0, 0, 0};
+    }
+    const uint16_t *arow = in + (lane & 15) * ldin + kq;
     for (int k0 = 0; k0 < K; k0 += 16) {
-        const int kk = k0 + 4 * (lane >> 4);
-        v4s bf[4];
+        const bool more = k0 + 16 < K;
+        v4s nb[4];
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-            const int n = n0 + ni * 16 + (lane & 15);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bf[ni][j] = (n < N && kk + j < K) ? (short)W[(long long)n * K + kk + j] : (short)0;
-        }
+        for (int ni = 0; ni < 4; ++ni)
+            nb[ni] = (ok[ni] && more) ? *reinterpret_cast<const v4s *>(wrow[ni] + k0 + 16) : v4s{0, 0, 0, 0};
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi) {
-            v4s af;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) af[j] = (short)in[(mi * 16 + (lane & 15)) * ldin + kk + j];
+            const v4s af = *reinterpret_cast<const v4s *>(arow + mi * 16 * ldin + k0);
 #pragma unroll
             for (int ni = 0; ni < 4; ++ni)
                 acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[ni], acc[mi][ni], 0, 0, 0);
         }
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) bf[ni] = nb[ni];
     }
 }
 
@@ -389,8 +400,15 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
         __syncthreads();
     }
 
-    // output layer in chunks of 256 columns with an online softmax (top two) per row
+    // output layer in chunks of 256 columns: each lane keeps a running softmax state (top two
+    // and the sum) for each of its 16 rows over the columns it owns; the 16 lanes of a row and
+    // the 4 waves combine once at the end
     const int NO = a.C + 1;
+    Soft ts[4][4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ts[mi][j] = soft_empty();
     for (int c0 = 0; c0 < NO; c0 += 256) {
         const int n0 = c0 + wave * 64;
         tile_layer(H2, ldh, a.hidden, a.W3, NO, n0, acc);
@@ -399,7 +417,6 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int r = mi * 16 + 4 * (lane >> 4) + j;
-                Soft t = soft_empty();
 #pragma unroll
                 for (int ni = 0; ni < 4; ++ni) {
                     const int n = n0 + ni * 16 + (lane & 15);
@@ -407,50 +424,67 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
                     const float v = acc[mi][ni][j] + a.b3[n];
                     if (a.logits && r0 + r < a.batch) a.logits[(long long)(r0 + r) * NO + n] = v;
                     if (n == a.C) stp[r] = v;
-                    else soft_add(t, v, n);
+                    else soft_add(ts[mi][j], v, n);
                 }
-                // combine the 16 lanes that share this row
-#pragma unroll
-                for (int o = 1; o < 16; o <<= 1) {
-                    Soft u;
-                    u.v1 = __shfl_xor(t.v1, o);
-                    u.v2 = __shfl_xor(t.v2, o);
-                    u.s = __shfl_xor(t.s, o);
-                    u.a1 = __shfl_xor(t.a1, o);
-                    u.a2 = __shfl_xor(t.a2, o);
-                    t = soft_merge(t, u);
-                }
-                if ((lane & 15) == 0) part[wave * IQ + r] = t;
             }
-        __syncthreads();
-        if (tid < IQ) {
-            Soft t = run[tid];
-            for (int w = 0; w < 4; ++w) t = soft_merge(t, part[w * IQ + tid]);
-            run[tid] = t;
-        }
-        __syncthreads();
     }
-    if (tid < IQ && r0 + tid < a.batch) {
-        const Soft t = run[tid];
-        a.top2[2 * (r0 + tid)] = t.a1;
-        a.top2[2 * (r0 + tid) + 1] = t.a2;
-        float mass;
-        if (a.nbr) {
-            // softmax mass of the best pivot's neighbourhood: its logits recomputed from the
-            // last hidden layer (one thread per row)
-            mass = 0.0f;
-            for (int e = 0; e < a.n_nbr; ++e) {
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = mi * 16 + 4 * (lane >> 4) + j;
+            Soft t = ts[mi][j];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                Soft u;
+                u.v1 = __shfl_xor(t.v1, o);
+                u.v2 = __shfl_xor(t.v2, o);
+                u.s = __shfl_xor(t.s, o);
+                u.a1 = __shfl_xor(t.a1, o);
+                u.a2 = __shfl_xor(t.a2, o);
+                t = soft_merge(t, u);
+            }
+            if ((lane & 15) == 0) part[wave * IQ + r] = t;
+        }
+    __syncthreads();
+    if (tid < IQ) {
+        Soft t = soft_empty();
+        for (int w = 0; w < 4; ++w) t = soft_merge(t, part[w * IQ + tid]);
+        run[tid] = t;
+    }
+    __syncthreads();
+    // confidence: softmax mass of the best pivot's neighbourhood, its logits recomputed from
+    // the last hidden layer by 4 threads per row (entries e = quarter, quarter + 4, ...);
+    // without neighbourhoods, the mass of the top two
+    {
+        const int row = tid >> 2, quarter = tid & 3;
+        const Soft t = run[row];
+        float mass = 0.0f;
+        if (a.nbr && t.a1 >= 0 && t.a1 < a.C) {  // (no valid best pivot: non-finite logits)
+            for (int e = quarter; e < a.n_nbr; e += 4) {
                 const int p = a.nbr[(long long)t.a1 * a.n_nbr + e];
-                float z = a.b3[p];
                 const uint16_t *w = a.W3 + (long long)p * a.hidden;
-                for (int h = 0; h < a.hidden; ++h) z += bf2f(H2[tid * ldh + h]) * bf2f(w[h]);
+                const uint16_t *hrow = H2 + row * ldh;
+                float z = a.b3[p];
+                for (int h = 0; h < a.hidden; h += 4) {
+                    const v4s wv = *reinterpret_cast<const v4s *>(w + h);
+                    const v4s hv = *reinterpret_cast<const v4s *>(hrow + h);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) z += bf2f((uint16_t)hv[q]) * bf2f((uint16_t)wv[q]);
+                }
                 mass += expf(fminf(z - t.v1, 0.0f));
             }
-        } else {
+        } else if (!a.nbr && quarter == 0) {
             mass = 1.0f + (t.v2 == -INFINITY ? 0.0f : expf(t.v2 - t.v1));
         }
-        a.conf[r0 + tid] = fminf(mass / t.s, 1.0f);
-        a.stop[r0 + tid] = stp[tid];
+        mass += __shfl_xor(mass, 1);
+        mass += __shfl_xor(mass, 2);
+        if (quarter == 0 && r0 + row < a.batch) {
+            a.top2[2 * (r0 + row)] = t.a1;
+            a.top2[2 * (r0 + row) + 1] = t.a2;
+            a.conf[r0 + row] = fminf(mass / t.s, 1.0f);
+            a.stop[r0 + row] = stp[row];
+        }
     }
 }
 
@@ -484,7 +518,7 @@ static int refresh_bf16(song_mlp *m, hipStream_t st) {
 }
 
 extern "C" int song_mlp_create(int in_dim, int hidden, int n_classes, int max_batch, uint64_t seed, song_mlp **out) {
-    if (!out || in_dim < 1 || hidden < 16 || n_classes < 2 || max_batch < 1)
+    if (!out || in_dim < 1 || hidden < 16 || hidden % 16 != 0 || n_classes < 2 || max_batch < 1)
         return mfail(-1, "bad MLP shape in=%d hidden=%d classes=%d batch=%d", in_dim, hidden, n_classes, max_batch);
     song_mlp *m = new song_mlp();
     m->in_dim = in_dim;

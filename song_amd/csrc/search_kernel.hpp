@@ -78,6 +78,7 @@ struct SearchParams {
     // queries whose visited set or queue overflowed are appended here for a re-run
     uint32_t *retry_list;
     uint32_t *retry_count;
+    int wave_prio;          // persistent kernel: 1 = raise the waves' issue priority (s_setprio 2)
 };
 
 // ---------------------------------------------------------------- keys
@@ -776,6 +777,9 @@ __global__ __launch_bounds__(256) void search_persistent(SearchParams P) {
     const int slot = (int)blockIdx.x * (int)(blockDim.x >> 6) + w;
     unsigned char *mine = smem + (size_t)w * P.wave_lds;
     const uint32_t count = *P.work_count;
+    // a tier whose searches are the longest runs at raised issue priority, so that waves of
+    // concurrently running tiers on the same SIMDs do not stretch its critical path
+    if (P.wave_prio) __builtin_amdgcn_s_setprio(2);
     for (;;) {
         // Every lane issues the add (lane 0 adds 1, the others 0) and lane 0's old value is
         // broadcast.  No lane-0 branch here: the query's lane-0 epilogue ends the previous

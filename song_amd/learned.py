@@ -337,7 +337,7 @@ class LearnedBatch:
     CUDA/HIP graph and replay() relaunches it."""
 
     def __init__(self, ls, nq, pq_learned, pq_fallback, k=10, conf_threshold=0.9, step_scale=1.5, step_add=8,
-                 storage=0, mode=MODE_FP32, tiers=None, k_ld=None):
+                 storage=0, mode=MODE_FP32, tiers=None, k_ld=None, **batch_kw):
         import torch
 
         self.ls, self.nq, self.k = ls, nq, k
@@ -346,7 +346,7 @@ class LearnedBatch:
         dev = ls.dev
         lo, hi = sorted({self.pq_l, self.pq_f}) if self.pq_l != self.pq_f else (self.pq_l, None)
         self.tiers = tiers or ([tier(lo)] + ([tier(hi)] if hi else []))
-        self.batch = Batch(ls.index, self.tiers, nq, mode=mode, storage=storage)
+        self.batch = Batch(ls.index, self.tiers, nq, mode=mode, storage=storage, **batch_kw)
         self.k_ld = k_ld or k
         self.ids = torch.empty((nq, self.k_ld), dtype=torch.int32, device=dev)
         self.flags = torch.empty(nq, dtype=torch.int32, device=dev)
