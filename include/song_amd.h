@@ -54,6 +54,8 @@ enum {
     SONG_FLAG_VISITED_OVF = 1u, /* visited table filled (query re-run with a larger table) */
     SONG_FLAG_QUEUE_OVF = 2u,   /* tie group at the pq-th distance exceeded the slack */
     SONG_FLAG_STEP_CAP = 4u,    /* hit the 1e6 expansion cap of graph.h:183 (same as the CPU) */
+    SONG_FLAG_BAD_LAUNCH = 0x80000000u, /* internal: the launch's LDS slice does not fit the
+                                            kernel's layout; the query was not searched */
 };
 
 typedef struct song_index song_index;

@@ -6,17 +6,23 @@
 // is strictly below the next candidate's (graph.h:188), mark every discovered neighbour
 // visited (graph.h:198-205), return the expanded list ascending by (dist, id).
 //
-// The CPU's unbounded queue is replaced by a bounded sorted array that only ever drops
-// entries that can never be expanded: an entry c is dropped when the pq-best expanded list
-// is full with largest distance < c.d, or when the queue already holds pq entries with
-// distance < c.d.  Entries tied with the pq-th distance are kept (up to `slack`), so the
-// expansion order -- and therefore the output -- equals the CPU's.
+// The CPU's unbounded queue is replaced by a bounded one that only ever drops entries that
+// can never be expanded: an entry c is dropped when the pq-best expanded list is full with
+// largest distance < c.d, or when the queue holds pq entries with distance < c.d.  Entries
+// tied with the pq-th distance are kept (up to `slack`), so the expansion order -- and
+// therefore the output -- equals the CPU's.  The queue is two-level: a sorted head of the
+// (up to) 64 smallest entries, from which the minimum is popped, and an unsorted bag of the
+// rest (every head entry <= T < every bag entry).  New candidates at most T are merged into
+// the head, the others appended to the bag; the head's overflow moves to the bag; an empty
+// head is refilled with the bag's 64 smallest (exact selection); the bag is bounded lazily
+// (the drops above are applied when it nears its capacity).
 //
 // Per-wave LDS layout (bytes, in order):
 //   visited  : H x u32 open-addressing set of (id + 1), probed in groups of 4, no deletion
-//   queue    : qcap = pq + slack + 128 keys, sorted ascending, live range [h, h + nq), h < 64
+//   head     : 160 keys, sorted ascending, live range [h, h + nq), nq <= 64
+//   bag      : qcap = pq + slack + 128 keys, unsorted, first bn live
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
-//   cid      : 64 u32    candidate ids of the current expansion
+//   cid      : 256 u32   candidate ids of the current expansion / radix-select bins
 //   qv       : qld f32   the query, zero-padded to a multiple of 16
 //
 // Each wave owns its LDS slice (one wave per workgroup in search_kernel; 1-4 waves, each
@@ -40,7 +46,7 @@ namespace song {
 
 enum { M_L2 = 0, M_IP = 1, M_COS = 2 };
 
-enum : uint32_t { F_VIS_OVF = 1u, F_Q_OVF = 2u, F_STEP_CAP = 4u };
+enum : uint32_t { F_VIS_OVF = 1u, F_Q_OVF = 2u, F_STEP_CAP = 4u, F_BAD_LAUNCH = 0x80000000u };
 
 struct SearchParams {
     const float *data;      // n x ld
@@ -74,7 +80,7 @@ struct SearchParams {
     // and search query qidx[t] (t itself without a list)
     uint32_t *work_next;
     const uint32_t *work_count;
-    uint32_t wave_lds;      // LDS bytes per wave (persistent kernel, several waves per group)
+    uint32_t wave_lds;      // LDS bytes of one wave's slice (every launch; the layout must fit)
     // queries whose visited set or queue overflowed are appended here for a re-run
     uint32_t *retry_list;
     uint32_t *retry_count;
@@ -112,6 +118,7 @@ __device__ __forceinline__ uint64_t rl64(uint64_t v, int lane) {
 }
 
 struct KeyF {
+    static constexpr int DKBITS = 32;  // significant bits of dk()
     uint64_t v;
     __device__ static KeyF make_f(float d, uint32_t id) { return KeyF{((uint64_t)ord32(d) << 32) | id}; }
     __device__ static KeyF inf() { return KeyF{~0ull}; }
@@ -123,6 +130,7 @@ struct KeyF {
 __device__ __forceinline__ bool lt(const KeyF &a, const KeyF &b) { return a.v < b.v; }
 
 struct KeyD {
+    static constexpr int DKBITS = 64;
     uint64_t d;
     uint32_t i, pad;
     __device__ static KeyD make_d(double dist, uint32_t id) { return KeyD{ord64(dist), id, 0u}; }
@@ -429,6 +437,130 @@ __device__ __forceinline__ int prefix_count(const Key *arr, int n, Pred below) {
     return lo + __popcll(ballot(v && below(e)));
 }
 
+// ---------------------------------------------------------------- wave-wide key sets
+
+__device__ __forceinline__ KeyF shfl_key(const KeyF &k, int src) {
+    return KeyF{((uint64_t)(uint32_t)__shfl((int)(k.v >> 32), src) << 32) | (uint32_t)__shfl((int)k.v, src)};
+}
+__device__ __forceinline__ KeyD shfl_key(const KeyD &k, int src) {
+    return KeyD{((uint64_t)(uint32_t)__shfl((int)(k.d >> 32), src) << 32) | (uint32_t)__shfl((int)k.d, src),
+                (uint32_t)__shfl((int)k.i, src), 0u};
+}
+template <class Key> __device__ __forceinline__ Key kmin(const Key &a, const Key &b) { return lt(b, a) ? b : a; }
+template <class Key> __device__ __forceinline__ Key kmax(const Key &a, const Key &b) { return lt(b, a) ? a : b; }
+
+// Bitonic merge steps j = J/2 .. 1 over one key per lane: within blocks of K lanes the
+// sequence ascends where (lane & K) == 0 and descends otherwise (K = 64: ascending).
+template <class Key>
+__device__ __forceinline__ Key bitonic_steps(Key v, int K, int J) {
+    const int lane = lane_id();
+    const bool up = (lane & K) == 0;
+    for (int j = J >> 1; j > 0; j >>= 1) {
+        const Key o = shfl_key(v, lane ^ j);
+        v = (((lane & j) == 0) == up) ? kmin(v, o) : kmax(v, o);
+    }
+    return v;
+}
+
+// The c = min(64, n) smallest keys of arr[0..n) (any order), ascending in lanes 0..c-1 with
+// +inf above.  Each 64-key block is sorted (bitonic) and merged into the running best 64:
+// min(best[i], block[63 - i]) holds the 64 smallest of both as a bitonic sequence.
+template <class Key>
+__device__ __forceinline__ Key select64(const Key *arr, int n) {
+    const int lane = lane_id();
+    Key best = Key::inf();
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        Key v = (b0 + lane < n) ? arr[b0 + lane] : Key::inf();
+        for (int k = 2; k <= 64; k <<= 1) v = bitonic_steps(v, k, k);
+        best = bitonic_steps(kmin(best, shfl_key(v, 63 - lane)), 64, 64);
+    }
+    return best;
+}
+
+// The r-th smallest (1-based, r <= n) distance key of arr[0..n), radix select on 8-bit digits
+// from the top; hist: 256 u32 of this wave's LDS.
+template <class Key>
+__device__ uint64_t select_dk(const Key *arr, int n, int r, uint32_t *hist) {
+    const int lane = lane_id();
+    uint64_t prefix = 0, pmask = 0;
+    uint32_t need = (uint32_t)r;
+    for (int shift = Key::DKBITS - 8; shift >= 0; shift -= 8) {
+        for (int b = lane; b < 256; b += 64) hist[b] = 0u;
+        lds_fence();
+        for (int i = lane; i < n; i += 64) {
+            const uint64_t d = arr[i].dk();
+            if ((d & pmask) == prefix) atomicAdd(&hist[(uint32_t)(d >> shift) & 255u], 1u);
+        }
+        lds_fence();
+        // lane l holds digits 4l..4l+3; an inclusive scan over the lanes finds the digit
+        // where the count of smaller entries reaches `need`
+        const uint32_t c0 = hist[4 * lane], c1 = hist[4 * lane + 1], c2 = hist[4 * lane + 2],
+                       c3 = hist[4 * lane + 3];
+        const uint32_t sum = c0 + c1 + c2 + c3;
+        uint32_t incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = (uint32_t)__shfl_up((int)incl, o);
+            if (lane >= o) incl += t;
+        }
+        const uint32_t excl = incl - sum;
+        const int L = __builtin_ctzll(ballot(excl < need && incl >= need));
+        uint32_t below = rl32(excl, L);
+        int digit = 4 * L;
+        const uint32_t d0 = rl32(c0, L), d1 = rl32(c1, L), d2 = rl32(c2, L);
+        if (below + d0 < need) {
+            below += d0;
+            ++digit;
+            if (below + d1 < need) {
+                below += d1;
+                ++digit;
+                if (below + d2 < need) {
+                    below += d2;
+                    ++digit;
+                }
+            }
+        }
+        need -= below;
+        prefix |= (uint64_t)digit << shift;
+        pmask |= 255ull << shift;
+        lds_fence();
+    }
+    return prefix;
+}
+
+// In-place compactions of arr[0..n) (order not kept); return the new count.  Entries only
+// move left, and a block is read before any of its slots is written.
+template <class Key>
+__device__ __forceinline__ int keep_dk_upto(Key *arr, int n, uint64_t dmax) {
+    const int lane = lane_id();
+    int out = 0;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        const bool v = b0 + lane < n;
+        const Key e = v ? arr[b0 + lane] : Key::inf();
+        const bool keep = v && e.dk() <= dmax;
+        const uint64_t km = ballot(keep);
+        if (keep) arr[out + lanes_below(km)] = e;
+        out += __popcll(km);
+    }
+    lds_fence();
+    return out;
+}
+template <class Key>
+__device__ __forceinline__ int keep_above(Key *arr, int n, const Key &kx) {
+    const int lane = lane_id();
+    int out = 0;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        const bool v = b0 + lane < n;
+        const Key e = v ? arr[b0 + lane] : Key::inf();
+        const bool keep = v && lt(kx, e);
+        const uint64_t km = ballot(keep);
+        if (keep) arr[out + lanes_below(km)] = e;
+        out += __popcll(km);
+    }
+    lds_fence();
+    return out;
+}
+
 // shapes: (lanes per candidate row, 16-byte units per lane) by units per row
 template <int SHAPE> struct Shape;
 template <> struct Shape<0> { static constexpr int SUB = 4, NV = 2; };    // <= 8 units
@@ -445,22 +577,30 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                                              unsigned char *smem) {
     using Key = typename KeySel<EXACT>::T;
     constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
-    constexpr int MB = 4;  // queue blocks moved per batch of LDS reads
+    constexpr int MB = 4;      // queue blocks moved per batch of LDS reads
+    constexpr int QH = 64;     // head entries kept sorted
+    constexpr int QCAP = 160;  // head buffer: QH + 64 of head room + 32 of merge room
     const int lane = lane_id();
     lds_fence();
     // per-query budgets are clamped to the launch's LDS sizing (pq) and output rows (out_ld)
     const int pq = P.pq_arr ? max(1, min(P.pq_arr[qi], P.pq)) : P.pq;
     const int k = max(0, min(min(P.k_arr ? P.k_arr[qi] : P.k, pq), P.out_ld));
-    const int capq = pq + P.slack;
     const uint32_t H = 1u << P.hlog;
     const uint32_t hmask = H - 1u;
     const uint32_t vis_limit = (H >> 2) * 3u;
 
     uint32_t *vis = reinterpret_cast<uint32_t *>(smem);
-    Key *Q = reinterpret_cast<Key *>(smem + (size_t)H * 4u);
-    Key *B = Q + P.qcap;
-    uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));
-    float *qv = reinterpret_cast<float *>(cid + 64);
+    Key *Q = reinterpret_cast<Key *>(smem + (size_t)H * 4u);  // sorted head, QCAP keys
+    Key *Bg = Q + QCAP;                                          // bag, qcap keys
+    Key *B = Bg + P.qcap;
+    uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));  // 256 u32
+    float *qv = reinterpret_cast<float *>(cid + 256);
+    // the layout must fit the slice the launch allocated: otherwise flag the query and touch
+    // no LDS (a host / kernel mismatch becomes an error instead of out-of-range accesses)
+    if (reinterpret_cast<unsigned char *>(qv + P.qld) - smem > (ptrdiff_t)P.wave_lds) {
+        if (lane == 0 && P.out_flags) P.out_flags[qi] = F_BAD_LAUNCH;
+        return;
+    }
 
     for (uint32_t i = lane; i < (H >> 2); i += 64) reinterpret_cast<uint4 *>(vis)[i] = make_uint4(0u, 0u, 0u, 0u);
     for (int i = lane; i < P.qld; i += 64) qv[i] = (i < P.dim) ? P.queries[(size_t)qi * P.dim + i] : 0.0f;
@@ -516,10 +656,25 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     uint32_t pf_id = 0xFFFFFFFFu;  // vertex whose adjacency row sits in pf_row
     uint32_t pf_row = 0u;
 
-    while (nq > 0) {
+    int bn = 0;          // bag entries: unsorted, all above T
+    Key T = Key::inf();  // head entries <= T < bag entries (+inf while the bag is empty)
+    while (nq > 0 || bn > 0) {
         if (steps >= max_steps) {
             flags |= F_STEP_CAP;
             break;
+        }
+        if (nq == 0) {
+            // refill the head with the (up to) 64 smallest bag entries
+            const int c = min(QH, bn);
+            const Key sel = select64(Bg, bn);
+            if (lane < c) Q[lane] = sel;
+            const Key top = sel.rl(c - 1);
+            h = 0;
+            nq = c;
+            lds_fence();
+            bn = keep_above(Bg, bn, top);
+            T = bn > 0 ? top : Key::inf();
+            pf_id = 0xFFFFFFFFu;
         }
         const Key now = Q[h];
         // stop rule, graph.h:188: |best| == pq and best.max.d < now.d (distance only, strict)
@@ -614,81 +769,101 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         const uint32_t cand = (lane < m) ? cid[lane] : 0u;
         const Key mine = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, m, cand, cid, qv, lenv);
 
-        // ---- merge the candidates into the queue
-        uint64_t cut = (nbest == pq) ? bmax : ~0ull;  // drop distance keys above cut
-        if (nq >= pq) cut = min(cut, Q[h + pq - 1].dk());
+        // ---- the candidates: at most T into the sorted head, above T into the bag
+        const uint64_t cut = (nbest == pq) ? bmax : ~0ull;  // drop distance keys above cut
         const bool surv = lane < m && mine.dk() <= cut;
-        const uint64_t sm = ballot(surv);
+        if (ballot(surv) == 0ull) continue;
+        if (bn + 62 > P.qcap) {
+            // make room: drop bag entries beyond the pq-th smallest distance of head + bag
+            // (ties kept) or beyond bmax; none of them can be expanded before the stop rule
+            uint64_t dcut = (nbest == pq) ? bmax : ~0ull;
+            if (pq <= nq) dcut = min(dcut, Q[h + pq - 1].dk());
+            else if (pq - nq <= bn) dcut = min(dcut, select_dk(Bg, bn, pq - nq, cid));
+            bn = keep_dk_upto(Bg, bn, dcut);
+            if (bn == 0) T = Key::inf();
+            if (bn + 62 > P.qcap) {  // the tie group at the pq-th distance exceeds the slack
+                flags |= F_Q_OVF;
+                break;
+            }
+        }
+        const bool to_b = surv && lt(T, mine);
+        const bool to_q = surv && !to_b;
+        const uint64_t bm = ballot(to_b);
+        if (to_b) Bg[bn + lanes_below(bm)] = mine;
+        bn += __popcll(bm);
+        const uint64_t sm = ballot(to_q);
         const int ms = __popcll(sm);
-        if (ms == 0) continue;
-
-        if (h >= 64) {
-            // move the live range back to the buffer start (left moves, first blocks first;
-            // MB blocks are read before any of them is written: sources lie at or beyond the
-            // targets of the same and earlier batches)
-            for (int blk = 0; blk * 64 < nq; blk += MB) {
-                Key e[MB];
+        if (ms > 0) {
+            if (h >= 64) {
+                // move the live range back to the buffer start (left moves, first blocks first;
+                // MB blocks are read before any of them is written: sources lie at or beyond the
+                // targets of the same and earlier batches)
+                for (int blk = 0; blk * 64 < nq; blk += MB) {
+                    Key e[MB];
 #pragma unroll
-                for (int c = 0; c < MB; ++c) e[c] = Q[h + min((blk + c) * 64 + lane, nq - 1)];
+                    for (int c = 0; c < MB; ++c) e[c] = Q[h + min((blk + c) * 64 + lane, nq - 1)];
 #pragma unroll
-                for (int c = 0; c < MB; ++c)
-                    if ((blk + c) * 64 + lane < nq) Q[(blk + c) * 64 + lane] = e[c];
+                    for (int c = 0; c < MB; ++c)
+                        if ((blk + c) * 64 + lane < nq) Q[(blk + c) * 64 + lane] = e[c];
+                }
+                h = 0;
+                lds_fence();
             }
-            h = 0;
-            lds_fence();
-        }
-        Key smin = Key::inf();
-        int rank = 0;  // survivors below this lane's survivor
-        for (uint64_t bits = sm; bits; bits &= bits - 1) {
-            const Key o = mine.rl(__builtin_ctzll(bits));
-            if (lt(o, smin)) smin = o;
-            rank += lt(o, mine) ? 1 : 0;
-        }
-        // entries [0, p0) stay; entries [p0, nq) move right by the survivors below them.
-        // Blocks are processed last first, MB at a time: the batch's reads are all issued
-        // before its writes, and a right shift only writes at or beyond the batch's own
-        // blocks, which later (lower) batches never read.
-        const int p0 = prefix_count(Q + h, nq, [&](const Key &e) { return lt(e, smin); });
-        int acc = 0;  // this lane's survivor: queue entries in [p0, nq) below it
-        const int bfirst = p0 >> 6;
-        for (int top = (nq - 1) >> 6; nq > p0 && top >= bfirst; top -= MB) {
-            Key e[MB];
-            bool live[MB];
-            int cnt[MB];
-#pragma unroll
-            for (int c = 0; c < MB; ++c) {
-                const int i = (top - c) * 64 + lane;
-                live[c] = i >= p0 && i < nq;
-                e[c] = live[c] ? Q[h + i] : Key::inf();
-                cnt[c] = 0;
-            }
+            Key smin = Key::inf();
+            int rank = 0;  // head-bound survivors below this lane's survivor
             for (uint64_t bits = sm; bits; bits &= bits - 1) {
-                const int j = __builtin_ctzll(bits);
-                const Key o = mine.rl(j);
-                int below = 0;
+                const Key o = mine.rl(__builtin_ctzll(bits));
+                if (lt(o, smin)) smin = o;
+                rank += lt(o, mine) ? 1 : 0;
+            }
+            // entries [0, p0) stay; entries [p0, nq) move right by the survivors below them.
+            // Blocks are processed last first, MB at a time: the batch's reads are all issued
+            // before its writes, and a right shift only writes at or beyond the batch's own
+            // blocks, which later (lower) batches never read.
+            const int p0 = prefix_count(Q + h, nq, [&](const Key &e) { return lt(e, smin); });
+            int acc = 0;  // this lane's survivor: head entries in [p0, nq) below it
+            const int bfirst = p0 >> 6;
+            for (int top = (nq - 1) >> 6; nq > p0 && top >= bfirst; top -= MB) {
+                Key e[MB];
+                bool live[MB];
+                int cnt[MB];
 #pragma unroll
                 for (int c = 0; c < MB; ++c) {
-                    const bool sb = lt(o, e[c]);
-                    cnt[c] += sb ? 1 : 0;
-                    below += __popcll(ballot(live[c] && !sb));
+                    const int i = (top - c) * 64 + lane;
+                    live[c] = i >= p0 && i < nq;
+                    e[c] = live[c] ? Q[h + i] : Key::inf();
+                    cnt[c] = 0;
                 }
-                if (lane == j) acc += below;
-            }
+                for (uint64_t bits = sm; bits; bits &= bits - 1) {
+                    const int j = __builtin_ctzll(bits);
+                    const Key o = mine.rl(j);
+                    int below = 0;
 #pragma unroll
-            for (int c = 0; c < MB; ++c)
-                if (live[c]) Q[h + (top - c) * 64 + lane + cnt[c]] = e[c];
-        }
-        if (surv) Q[h + p0 + acc + rank] = mine;
-        nq += ms;
-        lds_fence();
-
-        // truncate: keep entries with distance key <= cut2
-        uint64_t cut2 = (nbest == pq) ? bmax : ~0ull;
-        if (nq > pq) cut2 = min(cut2, Q[h + pq - 1].dk());
-        if (cut2 != ~0ull) nq = prefix_count(Q + h, nq, [&](const Key &e) { return e.dk() <= cut2; });
-        if (nq > capq) {
-            flags |= F_Q_OVF;
-            break;
+                    for (int c = 0; c < MB; ++c) {
+                        const bool sb = lt(o, e[c]);
+                        cnt[c] += sb ? 1 : 0;
+                        below += __popcll(ballot(live[c] && !sb));
+                    }
+                    if (lane == j) acc += below;
+                }
+#pragma unroll
+                for (int c = 0; c < MB; ++c)
+                    if (live[c]) Q[h + (top - c) * 64 + lane + cnt[c]] = e[c];
+            }
+            if (to_q) Q[h + p0 + acc + rank] = mine;
+            nq += ms;
+            lds_fence();
+            if (nq > QH) {
+                // the head keeps its QH smallest; the rest join the bag and T drops to the
+                // head's new largest entry
+                const int ov = nq - QH;
+                const Key e = (lane < ov) ? Q[h + QH + lane] : Key::inf();
+                T = Q[h + QH - 1];
+                if (lane < ov) Bg[bn + lane] = e;
+                bn += ov;
+                nq = QH;
+                lds_fence();
+            }
         }
         // the queue head may have changed: refresh the prefetch target only if it did
         if (nq > 0 && Q[h].id() != pf_id) {

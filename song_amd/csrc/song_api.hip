@@ -282,9 +282,12 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 // ---------------------------------------------------------------- launch planning
 
+// must match the per-wave layout of search_query: visited | head (160 keys) | bag (qcap keys) |
+// best (pq rounded to even) | 256 candidate ids / radix bins | query
 size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld) {
     size_t key = exact ? 16 : 8;
-    return ((size_t)4 << hlog) + (size_t)qcap * key + (size_t)((pq + 1) & ~1) * key + 64 * 4 + (size_t)qld * 4;
+    return ((size_t)4 << hlog) + (size_t)(160 + qcap) * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 +
+           (size_t)qld * 4;
 }
 
 int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape) {
@@ -327,6 +330,7 @@ static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned
     sp.gvis = gvis;
     sp.ghlog = kGhlog;
     sp.qld = p.qld;
+    sp.wave_lds = (uint32_t)p.lds;
     sp.data_lp = ix->d_lp[p.store];
     sp.scale = ix->d_scale[p.store];
     sp.lp_stride = ix->lp_stride[p.store];

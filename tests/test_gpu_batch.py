@@ -140,25 +140,32 @@ def test_serial_and_parallel_tiers_agree(c1):
 
 
 def test_batch_overflow_reruns_on_ties():
-    """Tie-heavy integer data with no slack and tiny tables: overflowed queries are re-run
-    by the retry kernel and the results still equal the reference's."""
+    """Tie-heavy integer data with tiny visited tables: queries that overflow them are re-run
+    by the retry kernel and the results equal the plain search's (and, at the golden pq,
+    the reference's)."""
     import torch
 
     meta, base, query, ref_ids, graph = golden_cases.load("tie_l2_3k")
     ix = song_amd.Index(base, graph, "l2")
     dev = _dev()
     q = torch.from_numpy(query).to(dev)
-    nq, pq = q.shape[0], meta["pq"]
-    b = song_amd.Batch(ix, [tier(pq, hash_log2=8, slack=0)], nq, mode=song_amd.MODE_EXACT, spill_log2=10,
-                       retry=tier(pq, hash_log2=12, slack=2048))
-    ids = torch.empty((nq, pq), dtype=torch.int32, device=dev)
-    fl = torch.empty(nq, dtype=torch.int32, device=dev)
-    b.run(q, ids, fl, pq=pq, k=pq)
-    counts = b.counts()
-    print(f"[batch] ties: re-runs {int(counts[-1])} of {nq}", flush=True)
-    assert counts[-1] > 0
-    assert int(fl.abs().max()) == 0
-    assert np.array_equal(ids.cpu().numpy().astype(np.int64), ref_ids)
+    nq = q.shape[0]
+    for pq in (meta["pq"], 400):
+        b = song_amd.Batch(ix, [tier(pq, hash_log2=8, slack=0)], nq, mode=song_amd.MODE_EXACT, spill_log2=10,
+                           retry=tier(pq, hash_log2=12, slack=2048))
+        ids = torch.empty((nq, pq), dtype=torch.int32, device=dev)
+        fl = torch.empty(nq, dtype=torch.int32, device=dev)
+        b.run(q, ids, fl, pq=pq, k=pq)
+        counts = b.counts()
+        print(f"[batch] ties pq={pq}: re-runs {int(counts[-1])} of {nq}", flush=True)
+        assert int(fl.abs().max()) == 0
+        want = torch.empty_like(ids)
+        ix.search_device(q, pq, pq, want, mode=song_amd.MODE_EXACT)
+        assert torch.equal(ids, want)
+        if pq == meta["pq"]:
+            assert np.array_equal(ids.cpu().numpy().astype(np.int64), ref_ids)
+        else:
+            assert counts[-1] > 0  # the visited tables overflowed: the re-run path ran
 
 
 def test_batch_graph_replays_follow_inputs(c1):
