@@ -66,6 +66,8 @@ struct Launch {
     size_t wave_lds = 0;
     uint32_t *gvis = nullptr;
     int ghlog = 13;  // spill table per wave: 2^ghlog u32
+    unsigned char *gbag = nullptr;  // queue bag per wave: bag_stride(plan) bytes
+    size_t bag_stride = 0;
     int pq_max = 0;
 };
 
@@ -96,7 +98,7 @@ struct song_batch {
     uint32_t *d_ctr = nullptr;  // count[T], next[T], retry count, retry next
     uint32_t *d_lists = nullptr;  // T x nq_max
     uint32_t *d_retry = nullptr;  // nq_max
-    std::vector<uint32_t *> allocs;
+    std::vector<void *> allocs;
     // parallel tiers: one stream per tier after the first and the fork / join events
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev;
@@ -169,7 +171,7 @@ extern "C" int song_batch_destroy(song_batch *b) {
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.exec);
     for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
     for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
-    for (uint32_t *p : b->allocs) (void)hipFree(p);
+    for (void *p : b->allocs) (void)hipFree(p);
     delete b;
     return SONG_OK;
 }
@@ -250,6 +252,10 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         if (hipMalloc(&L.gvis, slots * ((size_t)4 << L.ghlog)) != hipSuccess)
             return cleanup(fail(SONG_E_HIP, "tier %d: spill tables (%zu slots)", t, slots));
         b->allocs.push_back(L.gvis);
+        L.bag_stride = song::bag_stride(L.plan);
+        if (hipMalloc(&L.gbag, slots * L.bag_stride) != hipSuccess)
+            return cleanup(fail(SONG_E_HIP, "tier %d: queue bags (%zu slots)", t, slots));
+        b->allocs.push_back(L.gbag);
         b->tiers.push_back(L);
     }
     const int T = c.n_tiers;
@@ -362,6 +368,8 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
         p.lp_stride = ix->lp_stride[L.plan.store];
         p.gvis = L.gvis;
         p.ghlog = L.ghlog;
+        p.gbag = L.gbag;
+        p.gbag_stride = L.bag_stride;
         p.wave_lds = (uint32_t)L.wave_lds;
         // with tiers running side by side, the largest-pq tier (launched first, the longest
         // searches) gets issue priority

@@ -34,11 +34,13 @@ struct song_index {
     int integer_valued = 0;
     float vmin = 0.f, vmax = 0.f;
     size_t max_lds = 65536;
-    // visited spill tables, one per launch slot, one set per stream: calls on one stream
-    // run in order; calls on different streams use different tables
+    // visited spill tables and queue bags, one per launch slot, one set per stream: calls
+    // on one stream run in order; calls on different streams use different buffers
     struct Spill {
         uint32_t *d = nullptr;
         size_t slots = 0;
+        unsigned char *bag = nullptr;
+        size_t bag_bytes = 0;
     };
     std::unordered_map<void *, Spill> spill;
     std::mutex spill_mu;
@@ -57,6 +59,8 @@ struct Plan {
 };
 
 size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld);
+// bytes of one wave slot's queue bag (qcap keys, 256-byte aligned)
+size_t bag_stride(const Plan &p);
 // shape < 0: by row width; otherwise the given lanes-per-row shape
 int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape = -1);
 int default_hlog(int pq);

@@ -20,10 +20,13 @@
 // Per-wave LDS layout (bytes, in order):
 //   visited  : H x u32 open-addressing set of (id + 1), probed in groups of 4, no deletion
 //   head     : 160 keys, sorted ascending, live range [h, h + nq), nq <= 64
-//   bag      : qcap = pq + slack + 128 keys, unsorted, first bn live
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
 //   cid      : 256 u32   candidate ids of the current expansion / radix-select bins
 //   qv       : qld f32   the query, zero-padded to a multiple of 16
+// The bag (qcap = pq + slack + 128 keys, unsorted, first bn live) is in global memory, one
+// region per wave slot: it is written by plain stores as candidates arrive and read only when
+// the head runs empty or the bag nears its capacity, so LDS holds only what every step
+// touches and more waves fit on a CU.
 //
 // Each wave owns its LDS slice (one wave per workgroup in search_kernel; 1-4 waves, each
 // on its own slice, in search_persistent).  LDS instructions of a wave execute in issue
@@ -81,6 +84,8 @@ struct SearchParams {
     uint32_t *work_next;
     const uint32_t *work_count;
     uint32_t wave_lds;      // LDS bytes of one wave's slice (every launch; the layout must fit)
+    unsigned char *gbag;    // queue bags, one per launch slot / persistent wave, gbag_stride bytes
+    size_t gbag_stride;     // >= qcap keys
     // queries whose visited set or queue overflowed are appended here for a re-run
     uint32_t *retry_list;
     uint32_t *retry_count;
@@ -159,6 +164,9 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
 }
 // compiler-only fence for LDS traffic inside one wave (see the layout note above)
 __device__ __forceinline__ void lds_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+// before lanes read bag entries that other lanes of the wave stored to global memory: the
+// stores complete first (workgroup scope: the wave's own CU)
+__device__ __forceinline__ void bag_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
 
 // lowest set bit exponent of a nonzero float (value = odd * 2^e); INT_MAX for 0
 __device__ __forceinline__ int lsb_exp(float p) {
@@ -316,10 +324,10 @@ __device__ __forceinline__ Key eval_candidates(const SearchParams &P, int m, uin
         int mn = 0x3fffffff;
         for (int j0 = 0; j0 < nu; j0 += NVU * SUB) {
             uint4 u[NVU];
+            if (valid) {
+                // indices past the row are clamped (and skipped below): one predicate for all loads
 #pragma unroll
-            for (int v = 0; v < NVU; ++v) {
-                const int j = j0 + sl + v * SUB;
-                if (valid && j < nu) u[v] = row[j];
+                for (int v = 0; v < NVU; ++v) u[v] = row[min(j0 + sl + v * SUB, nu - 1)];
             }
 #pragma unroll
             for (int v = 0; v < NVU; ++v) {
@@ -469,8 +477,11 @@ template <class Key>
 __device__ __forceinline__ Key select64(const Key *arr, int n) {
     const int lane = lane_id();
     Key best = Key::inf();
+    Key nxt = (lane < n) ? arr[lane] : Key::inf();
     for (int b0 = 0; b0 < n; b0 += 64) {
-        Key v = (b0 + lane < n) ? arr[b0 + lane] : Key::inf();
+        Key v = nxt;
+        // the next block's load is in flight while this one is sorted
+        if (b0 + 64 < n) nxt = (b0 + 64 + lane < n) ? arr[b0 + 64 + lane] : Key::inf();
         for (int k = 2; k <= 64; k <<= 1) v = bitonic_steps(v, k, k);
         best = bitonic_steps(kmin(best, shfl_key(v, 63 - lane)), 64, 64);
     }
@@ -529,36 +540,31 @@ __device__ uint64_t select_dk(const Key *arr, int n, int r, uint32_t *hist) {
 }
 
 // In-place compactions of arr[0..n) (order not kept); return the new count.  Entries only
-// move left, and a block is read before any of its slots is written.
-template <class Key>
-__device__ __forceinline__ int keep_dk_upto(Key *arr, int n, uint64_t dmax) {
+// move left, below the start of the block being read, and each block's load (issued one
+// block ahead) completes before any of that block's slots is written.
+template <class Key, class Keep>
+__device__ __forceinline__ int compact(Key *arr, int n, Keep keep_fn) {
     const int lane = lane_id();
     int out = 0;
+    Key nxt = (lane < n) ? arr[lane] : Key::inf();
     for (int b0 = 0; b0 < n; b0 += 64) {
         const bool v = b0 + lane < n;
-        const Key e = v ? arr[b0 + lane] : Key::inf();
-        const bool keep = v && e.dk() <= dmax;
+        const Key e = nxt;
+        if (b0 + 64 < n) nxt = (b0 + 64 + lane < n) ? arr[b0 + 64 + lane] : Key::inf();
+        const bool keep = v && keep_fn(e);
         const uint64_t km = ballot(keep);
         if (keep) arr[out + lanes_below(km)] = e;
         out += __popcll(km);
     }
-    lds_fence();
     return out;
 }
 template <class Key>
+__device__ __forceinline__ int keep_dk_upto(Key *arr, int n, uint64_t dmax) {
+    return compact(arr, n, [&](const Key &e) { return e.dk() <= dmax; });
+}
+template <class Key>
 __device__ __forceinline__ int keep_above(Key *arr, int n, const Key &kx) {
-    const int lane = lane_id();
-    int out = 0;
-    for (int b0 = 0; b0 < n; b0 += 64) {
-        const bool v = b0 + lane < n;
-        const Key e = v ? arr[b0 + lane] : Key::inf();
-        const bool keep = v && lt(kx, e);
-        const uint64_t km = ballot(keep);
-        if (keep) arr[out + lanes_below(km)] = e;
-        out += __popcll(km);
-    }
-    lds_fence();
-    return out;
+    return compact(arr, n, [&](const Key &e) { return lt(kx, e); });
 }
 
 // shapes: (lanes per candidate row, 16-byte units per lane) by units per row
@@ -591,13 +597,14 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
 
     uint32_t *vis = reinterpret_cast<uint32_t *>(smem);
     Key *Q = reinterpret_cast<Key *>(smem + (size_t)H * 4u);  // sorted head, QCAP keys
-    Key *Bg = Q + QCAP;                                          // bag, qcap keys
-    Key *B = Bg + P.qcap;
+    Key *B = Q + QCAP;
     uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));  // 256 u32
     float *qv = reinterpret_cast<float *>(cid + 256);
-    // the layout must fit the slice the launch allocated: otherwise flag the query and touch
-    // no LDS (a host / kernel mismatch becomes an error instead of out-of-range accesses)
-    if (reinterpret_cast<unsigned char *>(qv + P.qld) - smem > (ptrdiff_t)P.wave_lds) {
+    Key *Bg = reinterpret_cast<Key *>(P.gbag + (size_t)slot_q * P.gbag_stride);  // bag, qcap keys
+    // the layout must fit the slice and the bag the launch allocated: otherwise flag the query
+    // and touch neither (a host / kernel mismatch becomes an error, not out-of-range accesses)
+    if (reinterpret_cast<unsigned char *>(qv + P.qld) - smem > (ptrdiff_t)P.wave_lds || !P.gbag ||
+        (size_t)P.qcap * sizeof(Key) > P.gbag_stride) {
         if (lane == 0 && P.out_flags) P.out_flags[qi] = F_BAD_LAUNCH;
         return;
     }
@@ -616,7 +623,11 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
 
     uint32_t flags = 0u;
     int steps = 0;
-    uint32_t nvis = 0u, adj_words = 0u, pf_hits = 0u;
+    uint32_t nvis = 0u;
+    // statistics counters live in lane 0 of a vector register (read once at the end), which
+    // keeps them out of the scalar registers the loop is short of
+    uint32_t adj_words = 0u, pf_hits = 0u;
+    const uint32_t l0 = lane == 0 ? 1u : 0u;
     // visited spill: once the LDS table nears its load limit, new ids go to a global table
     bool spill = false;
     uint32_t nglob = 0u;
@@ -651,7 +662,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         lds_fence();
     }
     const int max_steps = P.steps_arr ? min(P.steps_arr[qi], P.max_steps) : P.max_steps;
-    int settle = 0;
+    int settle = 0;  // lane 0 holds it
 
     uint32_t pf_id = 0xFFFFFFFFu;  // vertex whose adjacency row sits in pf_row
     uint32_t pf_row = 0u;
@@ -666,6 +677,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (nq == 0) {
             // refill the head with the (up to) 64 smallest bag entries
             const int c = min(QH, bn);
+            bag_fence();
             const Key sel = select64(Bg, bn);
             if (lane < c) Q[lane] = sel;
             const Key top = sel.rl(c - 1);
@@ -687,7 +699,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         uint32_t r;
         if (pf_id == now.id()) {
             r = pf_row;
-            ++pf_hits;
+            pf_hits += l0;
         } else {
             r = (lane < P.gstride) ? P.graph[(size_t)now.id() * (size_t)P.gstride + lane] : 0u;
         }
@@ -713,7 +725,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             const bool in_last = lb == 0 || (below0 & 1ull);
             const int p = in_last ? lb + __popcll(below0)
                                   : prefix_count(B, lb, [&](const Key &e) { return lt(e, now); });
-            if (p < k) settle = steps;
+            if (p < k && lane == 0) settle = steps;
             if (p < pq) {
                 const int last = min(nbest, pq - 1);  // entries [p, last) move to [p+1, last+1)
                 if (in_last) {
@@ -736,7 +748,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
 
         // neighbours of `now` in row order (graph.h:196-205)
         const int deg = (int)rl32(r, 0);
-        adj_words += 1u + (uint32_t)deg;
+        adj_words += l0 * (1u + (uint32_t)deg);
         if (!spill && nvis + 64u > vis_limit) {
             if (!P.gvis) {
                 flags |= F_VIS_OVF;
@@ -777,6 +789,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             // make room: drop bag entries beyond the pq-th smallest distance of head + bag
             // (ties kept) or beyond bmax; none of them can be expanded before the stop rule
             uint64_t dcut = (nbest == pq) ? bmax : ~0ull;
+            bag_fence();
             if (pq <= nq) dcut = min(dcut, Q[h + pq - 1].dk());
             else if (pq - nq <= bn) dcut = min(dcut, select_dk(Bg, bn, pq - nq, cid));
             bn = keep_dk_upto(Bg, bn, dcut);

@@ -55,6 +55,25 @@ static int ensure_gvis(song_index *ix, void *stream, size_t slots, uint32_t **ou
     return SONG_OK;
 }
 
+// queue bags: `stride` bytes (qcap keys) per launch slot, grown on demand like the tables
+static int ensure_gbag(song_index *ix, void *stream, size_t slots, size_t stride, unsigned char **out) {
+    std::lock_guard<std::mutex> g(ix->spill_mu);
+    song_index::Spill &sp = ix->spill[stream];
+    const size_t need = slots * stride;
+    if (sp.bag_bytes < need) {
+        if (sp.bag) {
+            HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+            HIP_TRY(hipFree(sp.bag));
+        }
+        sp.bag = nullptr;
+        sp.bag_bytes = 0;
+        HIP_TRY(hipMalloc(&sp.bag, need));
+        sp.bag_bytes = need;
+    }
+    *out = sp.bag;
+    return SONG_OK;
+}
+
 extern "C" const char *song_last_error(void) { return song::g_err.c_str(); }
 
 extern "C" void song_search_opts_default(song_search_opts *o) {
@@ -272,8 +291,10 @@ extern "C" int song_index_destroy(song_index *ix) {
     if (ix->d_data) (void)hipFree(ix->d_data);
     if (ix->d_graph) (void)hipFree(ix->d_graph);
     if (ix->d_norms) (void)hipFree(ix->d_norms);
-    for (auto &kv : ix->spill)
+    for (auto &kv : ix->spill) {
         if (kv.second.d) (void)hipFree(kv.second.d);
+        if (kv.second.bag) (void)hipFree(kv.second.bag);
+    }
     delete ix;
     return SONG_OK;
 }
@@ -282,13 +303,15 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 // ---------------------------------------------------------------- launch planning
 
-// must match the per-wave layout of search_query: visited | head (160 keys) | bag (qcap keys) |
-// best (pq rounded to even) | 256 candidate ids / radix bins | query
+// must match the per-wave layout of search_query: visited | head (160 keys) | best (pq
+// rounded to even) | 256 candidate ids / radix bins | query (the bag is in global memory)
 size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld) {
+    (void)qcap;
     size_t key = exact ? 16 : 8;
-    return ((size_t)4 << hlog) + (size_t)(160 + qcap) * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 +
-           (size_t)qld * 4;
+    return ((size_t)4 << hlog) + (size_t)160 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4;
 }
+
+size_t song::bag_stride(const Plan &p) { return ((size_t)p.qcap * (p.exact ? 16 : 8) + 255) & ~(size_t)255; }
 
 int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
@@ -308,9 +331,11 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
     return SONG_OK;
 }
 
-// 4096 slots (16 KB) hold the ~1-3k distinct vertices a search discovers up to pq ~ 1000
-// at the 3/4 load limit; queries that discover more are re-run with larger tables.
-int song::default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
+// LDS visited-table size by budget, the fastest on the 1M x 128 workload: small budgets
+// gain more from waves per CU (2^10 slots; most queries continue in the global table) than
+// they lose to the global table, pq ~ 768 runs best with 2^12 slots (the ~1-3k vertices a
+// search discovers stay in LDS).  Queries whose global table overflows are re-run.
+int song::default_hlog(int pq) { return pq <= 384 ? 10 : pq > 1024 ? 13 : 12; }
 
 
 SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
@@ -321,9 +346,13 @@ SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
     }
 }
 
-static int launch(const song_index *ix, const Plan &p, SearchParams sp, unsigned count, hipStream_t st,
+static int launch(song_index *ix, const Plan &p, SearchParams sp, unsigned count, hipStream_t st,
                   uint32_t *gvis) {
     SearchFn fn = search_fn(ix->metric, p);
+    const size_t stride = bag_stride(p);
+    int rc = ensure_gbag(ix, (void *)st, std::min(count, kChunk), stride, &sp.gbag);
+    if (rc != SONG_OK) return rc;
+    sp.gbag_stride = stride;
     sp.hlog = p.hlog;
     sp.slack = p.slack;
     sp.qcap = p.qcap;
@@ -412,7 +441,7 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
     int rcg = ensure_gvis(ix, stream, (size_t)std::min<uint64_t>(nrun ? nrun : 1, kChunk), &gvis);
     if (rcg != SONG_OK) return rcg;
     int hlog = opts.hash_log2 > 0 ? opts.hash_log2 : default_hlog(pq);
-    hlog = std::max(9, std::min(hlog, 16));
+    hlog = std::max(8, std::min(hlog, 16));
     Plan plan;
     int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan);
     if (rc != SONG_OK) return rc;
