@@ -110,11 +110,7 @@ struct song_batch {
     std::vector<Cached> graphs;
 };
 
-static int default_tier_hlog(int pq) {
-    if (pq <= 64) return 10;
-    if (pq <= 256) return 11;
-    return default_hlog(pq);
-}
+static int default_tier_hlog(int pq) { return default_hlog(pq); }
 
 extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     memset(c, 0, sizeof(*c));
