@@ -121,7 +121,8 @@ int song_search(song_index *index, const float *queries, uint64_t nq, int pq, in
  * float32 (nullable); d_flags nq uint32 (nullable); d_stats nq x 8 uint32
  * (expansions, distances computed, adjacency words read = sum of 1 + degree
  * over expanded vertices, adjacency-row prefetch hits, settle step = the last
- * expansion that changed the first k results, 0, 0, 0; nullable).  pq is the
+ * expansion that changed the first k results, queue-head refills from the bag,
+ * bag truncations, 0; nullable).  pq is the
  * maximum budget.
  * With opts->retry set (the default) the call reads the flags back, re-runs
  * overflowed queries and synchronizes the stream; with retry = 0 it only

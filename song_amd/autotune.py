@@ -98,21 +98,29 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         if best_ms is None:
             raise SongError(f"tier {ti}: default configuration failed: {ref}")
         trials = [(dict(best), best_ms)]
-        for key, choices in (("waves_per_group", (1, 2, 4)),
-                             ("hash_log2", tuple(range(8, 15))),
-                             ("shape", SHAPES),
-                             ("slack", (0, 8, 32, 128)),
-                             ("groups_per_cu", (0, 1, 2, 4, 8))):
-            cur = best[key]
-            if key == "hash_log2" and cur == 0:
-                cur = -1
-            for v in _neighbours(cur, choices):
-                cand = dict(best)
-                cand[key] = v
-                ms, _ = measure(cand, want=ref)
-                trials.append((dict(cand), ms))
-                if ms is not None and ms < best_ms:
-                    best, best_ms = cand, ms
+        # two rounds of coordinate descent: a knob's best value can depend on another's (a larger
+        # visited table only pays off with fewer waves per group, which the first round may have
+        # rejected at the smaller table)
+        for _round in range(2):
+            for key, choices in (("waves_per_group", (1, 2, 4)),
+                                 ("hash_log2", tuple(range(8, 15))),
+                                 ("shape", SHAPES),
+                                 ("slack", (0, 8, 32, 128)),
+                                 ("groups_per_cu", (0, 1, 2, 4, 8))):
+                cur = best[key]
+                if key == "hash_log2" and cur == 0:
+                    cur = -1
+                for v in _neighbours(cur, choices):
+                    cand = dict(best)
+                    cand[key] = v
+                    ms, _ = measure(cand, want=ref)
+                    if ms is None and key == "hash_log2" and cand["waves_per_group"] > 1:
+                        # a table too large for the group's LDS: try it with one wave per group
+                        cand["waves_per_group"] = 1
+                        ms, _ = measure(cand, want=ref)
+                    trials.append((dict(cand), ms))
+                    if ms is not None and ms < best_ms:
+                        best, best_ms = cand, ms
         tiers[ti] = best
         entry_log = {"tier": ti, "pq_max": t["pq_max"], "queries": int(nq), "ms": best_ms,
                      "qps": nq / (best_ms * 1e-3), "config": best,

@@ -19,7 +19,8 @@
 //
 // Per-wave LDS layout (bytes, in order):
 //   visited  : H x u32 open-addressing set of (id + 1), probed in groups of 4, no deletion
-//   head     : 160 keys, sorted ascending, live range [h, h + nq), nq <= 64
+//   scratch  : 64 keys    merge staging of the queue head (the head itself is one key per
+//                          lane in registers, ascending over lanes [h, h + nq), nq <= 64)
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
 //   cid      : 256 u32   candidate ids of the current expansion / radix-select bins
 //   qv       : qld f32   the query, zero-padded to a multiple of 16
@@ -66,7 +67,8 @@ struct SearchParams {
     float *out_dist;        // nq x out_ld or null
     uint32_t *out_flags;    // nq or null
     uint32_t *out_stats;    // nq x 8 or null: expansions, distances, sum(1+deg), prefetch hits,
-                            // settle step (last expansion that changed the first k), 0, 0, 0
+                            // settle step (last expansion that changed the first k), head
+                            // refills from the bag, bag truncations, 0
     uint64_t n;
     int dim, ld, gstride;
     int qld;                // query floats in LDS: dim padded to 16
@@ -583,9 +585,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                                              unsigned char *smem) {
     using Key = typename KeySel<EXACT>::T;
     constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
-    constexpr int MB = 4;      // queue blocks moved per batch of LDS reads
-    constexpr int QH = 64;     // head entries kept sorted
-    constexpr int QCAP = 160;  // head buffer: QH + 64 of head room + 32 of merge room
+    constexpr int QH = 64;  // head entries: one per lane, ascending from lane h
     const int lane = lane_id();
     lds_fence();
     // per-query budgets are clamped to the launch's LDS sizing (pq) and output rows (out_ld)
@@ -596,8 +596,8 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     const uint32_t vis_limit = (H >> 2) * 3u;
 
     uint32_t *vis = reinterpret_cast<uint32_t *>(smem);
-    Key *Q = reinterpret_cast<Key *>(smem + (size_t)H * 4u);  // sorted head, QCAP keys
-    Key *B = Q + QCAP;
+    Key *S = reinterpret_cast<Key *>(smem + (size_t)H * 4u);  // merge scratch, QH keys
+    Key *B = S + QH;
     uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));  // 256 u32
     float *qv = reinterpret_cast<float *>(cid + 256);
     Key *Bg = reinterpret_cast<Key *>(P.gbag + (size_t)slot_q * P.gbag_stride);  // bag, qcap keys
@@ -626,7 +626,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     uint32_t nvis = 0u;
     // statistics counters live in lane 0 of a vector register (read once at the end), which
     // keeps them out of the scalar registers the loop is short of
-    uint32_t adj_words = 0u, pf_hits = 0u;
+    uint32_t adj_words = 0u, pf_hits = 0u, refills = 0u, truncs = 0u;
     const uint32_t l0 = lane == 0 ? 1u : 0u;
     // visited spill: once the LDS table nears its load limit, new ids go to a global table
     bool spill = false;
@@ -635,6 +635,8 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     const uint32_t GH = 1u << P.ghlog, gmask = GH - 1u, glimit = (GH >> 2) * 3u;
     int h = 0, nq = 0, nbest = 0;
     uint64_t bmax = ~0ull;  // distance key of best[pq-1] once the list is full
+    // the queue head lives in registers: lanes [h, h + nq) hold its keys in ascending order
+    Key hq = Key::inf();
 
     // start vertices (graph.h:175-181: vertex 0; the learned form passes its own), each
     // entered once through the visited set and placed in the queue in (dist, id) order
@@ -657,7 +659,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         const Key k0 = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, m0, cand0, cid, qv, lenv);
         int rank = 0;
         for (int j = 0; j < m0; ++j) rank += lt(k0.rl(j), k0) ? 1 : 0;
-        if (lane < m0) Q[rank] = k0;
+        if (lane < m0) S[rank] = k0;
+        lds_fence();
+        hq = (lane < m0) ? S[lane] : Key::inf();
         nq = m0;
         lds_fence();
     }
@@ -677,18 +681,18 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (nq == 0) {
             // refill the head with the (up to) 64 smallest bag entries
             const int c = min(QH, bn);
+            refills += l0;
             bag_fence();
             const Key sel = select64(Bg, bn);
-            if (lane < c) Q[lane] = sel;
+            hq = sel;  // +inf in lanes >= c
             const Key top = sel.rl(c - 1);
             h = 0;
             nq = c;
-            lds_fence();
             bn = keep_above(Bg, bn, top);
             T = bn > 0 ? top : Key::inf();
             pf_id = 0xFFFFFFFFu;
         }
-        const Key now = Q[h];
+        const Key now = hq.rl(h);
         // stop rule, graph.h:188: |best| == pq and best.max.d < now.d (distance only, strict)
         if (nbest == pq && bmax < now.dk()) break;
         ++steps;
@@ -705,7 +709,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         }
         // prefetch the row of the new queue head, the likely next expansion
         if (nq > 0) {
-            pf_id = Q[h].id();
+            pf_id = hq.rl(h).id();
             pf_row = (lane < P.gstride) ? P.graph[(size_t)pf_id * (size_t)P.gstride + lane] : 0u;
         } else {
             pf_id = 0xFFFFFFFFu;
@@ -789,8 +793,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             // make room: drop bag entries beyond the pq-th smallest distance of head + bag
             // (ties kept) or beyond bmax; none of them can be expanded before the stop rule
             uint64_t dcut = (nbest == pq) ? bmax : ~0ull;
+            truncs += l0;
             bag_fence();
-            if (pq <= nq) dcut = min(dcut, Q[h + pq - 1].dk());
+            if (pq <= nq) dcut = min(dcut, hq.rl(h + pq - 1).dk());
             else if (pq - nq <= bn) dcut = min(dcut, select_dk(Bg, bn, pq - nq, cid));
             bn = keep_dk_upto(Bg, bn, dcut);
             if (bn == 0) T = Key::inf();
@@ -807,80 +812,44 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         const uint64_t sm = ballot(to_q);
         const int ms = __popcll(sm);
         if (ms > 0) {
-            if (h >= 64) {
-                // move the live range back to the buffer start (left moves, first blocks first;
-                // MB blocks are read before any of them is written: sources lie at or beyond the
-                // targets of the same and earlier batches)
-                for (int blk = 0; blk * 64 < nq; blk += MB) {
-                    Key e[MB];
-#pragma unroll
-                    for (int c = 0; c < MB; ++c) e[c] = Q[h + min((blk + c) * 64 + lane, nq - 1)];
-#pragma unroll
-                    for (int c = 0; c < MB; ++c)
-                        if ((blk + c) * 64 + lane < nq) Q[(blk + c) * 64 + lane] = e[c];
-                }
-                h = 0;
-                lds_fence();
-            }
-            Key smin = Key::inf();
-            int rank = 0;  // head-bound survivors below this lane's survivor
+            // merge into the register head.  A head key moves down by the h consumed lanes and
+            // up by the survivors below it; a survivor lands after the head keys and survivors
+            // below it.  Keys are distinct, so the destinations are a permutation of
+            // [0, nq + ms); those past the last lane go to the bag (they exceed every key that
+            // stays), the rest pass through the LDS scratch.
+            const bool live = lane >= h && lane < h + nq;
+            int cnt = 0, sdst = 0;
             for (uint64_t bits = sm; bits; bits &= bits - 1) {
-                const Key o = mine.rl(__builtin_ctzll(bits));
-                if (lt(o, smin)) smin = o;
-                rank += lt(o, mine) ? 1 : 0;
+                const int j = __builtin_ctzll(bits);
+                const Key o = mine.rl(j);
+                cnt += (live && lt(o, hq)) ? 1 : 0;
+                const int d = __popcll(ballot(live && lt(hq, o))) + __popcll(ballot(to_q && lt(mine, o)));
+                if (lane == j) sdst = d;
             }
-            // entries [0, p0) stay; entries [p0, nq) move right by the survivors below them.
-            // Blocks are processed last first, MB at a time: the batch's reads are all issued
-            // before its writes, and a right shift only writes at or beyond the batch's own
-            // blocks, which later (lower) batches never read.
-            const int p0 = prefix_count(Q + h, nq, [&](const Key &e) { return lt(e, smin); });
-            int acc = 0;  // this lane's survivor: head entries in [p0, nq) below it
-            const int bfirst = p0 >> 6;
-            for (int top = (nq - 1) >> 6; nq > p0 && top >= bfirst; top -= MB) {
-                Key e[MB];
-                bool live[MB];
-                int cnt[MB];
-#pragma unroll
-                for (int c = 0; c < MB; ++c) {
-                    const int i = (top - c) * 64 + lane;
-                    live[c] = i >= p0 && i < nq;
-                    e[c] = live[c] ? Q[h + i] : Key::inf();
-                    cnt[c] = 0;
-                }
-                for (uint64_t bits = sm; bits; bits &= bits - 1) {
-                    const int j = __builtin_ctzll(bits);
-                    const Key o = mine.rl(j);
-                    int below = 0;
-#pragma unroll
-                    for (int c = 0; c < MB; ++c) {
-                        const bool sb = lt(o, e[c]);
-                        cnt[c] += sb ? 1 : 0;
-                        below += __popcll(ballot(live[c] && !sb));
-                    }
-                    if (lane == j) acc += below;
-                }
-#pragma unroll
-                for (int c = 0; c < MB; ++c)
-                    if (live[c]) Q[h + (top - c) * 64 + lane + cnt[c]] = e[c];
+            const int hdst = lane - h + cnt;
+            const int nn = nq + ms;
+            if (nn > QH) {
+                if (live && hdst >= QH) Bg[bn + hdst - QH] = hq;
+                if (to_q && sdst >= QH) Bg[bn + sdst - QH] = mine;
             }
-            if (to_q) Q[h + p0 + acc + rank] = mine;
-            nq += ms;
+            if (live && hdst < QH) S[hdst] = hq;
+            if (to_q && sdst < QH) S[sdst] = mine;
             lds_fence();
-            if (nq > QH) {
-                // the head keeps its QH smallest; the rest join the bag and T drops to the
-                // head's new largest entry
-                const int ov = nq - QH;
-                const Key e = (lane < ov) ? Q[h + QH + lane] : Key::inf();
-                T = Q[h + QH - 1];
-                if (lane < ov) Bg[bn + lane] = e;
-                bn += ov;
+            hq = (lane < nn) ? S[lane] : Key::inf();
+            h = 0;
+            if (nn > QH) {
+                // the head keeps its QH smallest; T drops to its new largest key
+                bn += nn - QH;
                 nq = QH;
-                lds_fence();
+                T = hq.rl(QH - 1);
+            } else {
+                nq = nn;
             }
+            lds_fence();
         }
         // the queue head may have changed: refresh the prefetch target only if it did
-        if (nq > 0 && Q[h].id() != pf_id) {
-            pf_id = Q[h].id();
+        if (nq > 0 && hq.rl(h).id() != pf_id) {
+            pf_id = hq.rl(h).id();
             pf_row = (lane < P.gstride) ? P.graph[(size_t)pf_id * (size_t)P.gstride + lane] : 0u;
         }
     }
@@ -939,8 +908,8 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             o[2] = adj_words;
             o[3] = pf_hits;
             o[4] = (uint32_t)settle;
-            o[5] = 0u;
-            o[6] = 0u;
+            o[5] = refills;
+            o[6] = truncs;
             o[7] = 0u;
         }
     }

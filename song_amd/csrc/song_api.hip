@@ -303,12 +303,13 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 // ---------------------------------------------------------------- launch planning
 
-// must match the per-wave layout of search_query: visited | head (160 keys) | best (pq
-// rounded to even) | 256 candidate ids / radix bins | query (the bag is in global memory)
+// must match the per-wave layout of search_query: visited | head merge scratch (64 keys) |
+// best (pq rounded to even) | 256 candidate ids / radix bins | query (the head is in
+// registers, the bag in global memory)
 size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld) {
     (void)qcap;
     size_t key = exact ? 16 : 8;
-    return ((size_t)4 << hlog) + (size_t)160 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4;
+    return ((size_t)4 << hlog) + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4;
 }
 
 size_t song::bag_stride(const Plan &p) { return ((size_t)p.qcap * (p.exact ? 16 : 8) + 255) & ~(size_t)255; }
