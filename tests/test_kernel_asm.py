@@ -56,21 +56,31 @@ def loop_head_is_uniform(body):
     return "s_andn2_b64 exec, exec" not in "\n".join(body[max(0, d1[0] - 8):d1[0]])
 
 
-@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-def test_persistent_loop_is_wave_uniform():
+@pytest.fixture(scope="module")
+def search_asm():
+    """Assembly text of search_{l2,ip,cos}.hip for gfx950, compiled once for this module."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
     tmp = tempfile.mkdtemp(prefix="song_asm_")
     try:
         jobs = [_asm(m, tmp) for m in ("l2", "ip", "cos")]
-        kernels = {}
+        texts = []
         for p, out in jobs:
             _, err = p.communicate(timeout=600)
             assert p.returncode == 0, err.decode(errors="replace")[-2000:]
-            kernels.update(_functions(open(out).read(), r"_ZN4song17search_persistent"))
-        assert len(kernels) == 60
-        bad = [k for k, body in kernels.items() if not loop_head_is_uniform(body)]
-        assert not bad, f"{len(bad)} persistent kernels with a split work loop, e.g. {bad[:3]}"
+            texts.append(open(out).read())
+        yield texts
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def test_persistent_loop_is_wave_uniform(search_asm):
+    kernels = {}
+    for text in search_asm:
+        kernels.update(_functions(text, r"_ZN4song17search_persistent"))
+    assert len(kernels) == 60
+    bad = [k for k, body in kernels.items() if not loop_head_is_uniform(body)]
+    assert not bad, f"{len(bad)} persistent kernels with a split work loop, e.g. {bad[:3]}"
 
 
 def test_checker_flags_a_split_loop():
