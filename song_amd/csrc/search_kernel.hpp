@@ -46,6 +46,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace song {
 
 enum { M_L2 = 0, M_IP = 1, M_COS = 2 };
@@ -166,6 +168,11 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
 }
 // compiler-only fence for LDS traffic inside one wave (see the layout note above)
 __device__ __forceinline__ void lds_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+// wait for every vector-memory operation of this wave (s_waitcnt vmcnt(0)).  Placed at the end
+// of the rare bag paths (refill, truncation): their loads land in registers that the common path
+// reuses, and without a wait on the rare path itself the compiler has to wait for all memory
+// operations -- the adjacency prefetch included -- at the top of every expansion.
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 // before lanes read bag entries that other lanes of the wave stored to global memory: the
 // stores complete first (workgroup scope: the wave's own CU)
 __device__ __forceinline__ void bag_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
@@ -668,15 +675,28 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     const int max_steps = P.steps_arr ? min(P.steps_arr[qi], P.max_steps) : P.max_steps;
     int settle = 0;  // lane 0 holds it
 
-    uint32_t pf_id = 0xFFFFFFFFu;  // vertex whose adjacency row sits in pf_row
-    uint32_t pf_row = 0u;
+    // adjacency rows loaded one expansion ahead, in two slots used in turn: an expansion of
+    // parity X reads its row from slot X where it lies and loads the next head's row into slot
+    // Y, whose row the previous expansion has consumed.  No row is copied between registers, so
+    // nothing waits for a load before the row is first used (a copy of a register whose load is
+    // in flight waits for that load).  The loop body is instantiated once per parity.
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    uint32_t pid[2] = {NONE, NONE}, prow[2] = {0u, 0u};
+    // all lanes load (lanes past the row read its last word and are never used: a row's
+    // degree is below gstride), so the load needs no lane mask and no zero fill
+    const auto adj_row = [&](uint32_t v) -> uint32_t {
+        return P.graph[(size_t)v * (size_t)P.gstride + (uint32_t)min(lane, P.gstride - 1)];
+    };
 
     int bn = 0;          // bag entries: unsorted, all above T
     Key T = Key::inf();  // head entries <= T < bag entries (+inf while the bag is empty)
-    while (nq > 0 || bn > 0) {
+    // one expansion; returns true to leave the loop
+    const auto expand = [&](auto parity) -> bool {
+        constexpr int X = decltype(parity)::value, Y = 1 - X;
+        if (nq == 0 && bn == 0) return true;
         if (steps >= max_steps) {
             flags |= F_STEP_CAP;
-            break;
+            return true;
         }
         if (nq == 0) {
             // refill the head with the (up to) 64 smallest bag entries
@@ -690,30 +710,31 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             nq = c;
             bn = keep_above(Bg, bn, top);
             T = bn > 0 ? top : Key::inf();
-            pf_id = 0xFFFFFFFFu;
+            pid[X] = NONE;
+            vm_drain();
         }
         const Key now = hq.rl(h);
         // stop rule, graph.h:188: |best| == pq and best.max.d < now.d (distance only, strict)
-        if (nbest == pq && bmax < now.dk()) break;
+        if (nbest == pq && bmax < now.dk()) return true;
         ++steps;
         ++h;
         --nq;
 
-        // adjacency row of `now`: from the prefetch when it predicted this vertex
-        uint32_t r;
-        if (pf_id == now.id()) {
-            r = pf_row;
+        // adjacency row of `now`: slot X holds it when the prefetch predicted this vertex;
+        // otherwise it is loaded into slot X now
+        if (pid[X] == now.id()) {
             pf_hits += l0;
         } else {
-            r = (lane < P.gstride) ? P.graph[(size_t)now.id() * (size_t)P.gstride + lane] : 0u;
+            pid[X] = now.id();
+            prow[X] = adj_row(now.id());
         }
-        // prefetch the row of the new queue head, the likely next expansion
-        if (nq > 0) {
-            pf_id = hq.rl(h).id();
-            pf_row = (lane < P.gstride) ? P.graph[(size_t)pf_id * (size_t)P.gstride + lane] : 0u;
-        } else {
-            pf_id = 0xFFFFFFFFu;
-        }
+        const uint32_t r = prow[X];
+        // prefetch the row of the new queue head, the likely next expansion, into slot Y.  The
+        // load is issued on every expansion (vertex 0's row when the head is empty) so that a
+        // load always follows this expansion's row: the wait before the row's first use then
+        // leaves the prefetch in flight.
+        pid[Y] = nq > 0 ? hq.rl(h).id() : NONE;
+        prow[Y] = adj_row(nq > 0 ? pid[Y] : 0u);
 
         // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195):
         // find its position, shift the entries behind it one place (last block first)
@@ -756,7 +777,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (!spill && nvis + 64u > vis_limit) {
             if (!P.gvis) {
                 flags |= F_VIS_OVF;
-                break;
+                return true;
             }
             // switch to the global table for the rest of this query: clear it first
             spill = true;
@@ -777,10 +798,10 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             nglob += (uint32_t)m;
             if (nglob + 64u > glimit) {
                 flags |= F_VIS_OVF;
-                break;
+                return true;
             }
         }
-        if (m == 0) continue;
+        if (m == 0) return false;
         lds_fence();
         const uint32_t cand = (lane < m) ? cid[lane] : 0u;
         const Key mine = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, m, cand, cid, qv, lenv);
@@ -788,7 +809,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         // ---- the candidates: at most T into the sorted head, above T into the bag
         const uint64_t cut = (nbest == pq) ? bmax : ~0ull;  // drop distance keys above cut
         const bool surv = lane < m && mine.dk() <= cut;
-        if (ballot(surv) == 0ull) continue;
+        if (ballot(surv) == 0ull) return false;
         if (bn + 62 > P.qcap) {
             // make room: drop bag entries beyond the pq-th smallest distance of head + bag
             // (ties kept) or beyond bmax; none of them can be expanded before the stop rule
@@ -798,10 +819,11 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             if (pq <= nq) dcut = min(dcut, hq.rl(h + pq - 1).dk());
             else if (pq - nq <= bn) dcut = min(dcut, select_dk(Bg, bn, pq - nq, cid));
             bn = keep_dk_upto(Bg, bn, dcut);
+            vm_drain();
             if (bn == 0) T = Key::inf();
             if (bn + 62 > P.qcap) {  // the tie group at the pq-th distance exceeds the slack
                 flags |= F_Q_OVF;
-                break;
+                return true;
             }
         }
         const bool to_b = surv && lt(T, mine);
@@ -848,10 +870,15 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             lds_fence();
         }
         // the queue head may have changed: refresh the prefetch target only if it did
-        if (nq > 0 && hq.rl(h).id() != pf_id) {
-            pf_id = hq.rl(h).id();
-            pf_row = (lane < P.gstride) ? P.graph[(size_t)pf_id * (size_t)P.gstride + lane] : 0u;
+        if (nq > 0 && hq.rl(h).id() != pid[Y]) {
+            pid[Y] = hq.rl(h).id();
+            prow[Y] = adj_row(pid[Y]);
         }
+        return false;
+    };
+    for (;;) {
+        if (expand(std::integral_constant<int, 0>{})) break;
+        if (expand(std::integral_constant<int, 1>{})) break;
     }
 
     const int cnt = min(k, nbest);
