@@ -222,6 +222,12 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         Launch L;
         L.pq_max = ti.pq_max;
         int rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape);
+        // the re-run's default table (the largest tier's, 4x) may not fit one wave's LDS: take the
+        // largest that does (the re-run also has a 4x larger global spill table)
+        while (rc == SONG_E_OVERFLOW && is_retry && !ti.hash_log2 && hlog > top_hlog) {
+            --hlog;
+            rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape);
+        }
         if (rc != SONG_OK) return cleanup(rc);
         L.fn = search_fn(ix->metric, L.plan, true);
         L.waves = ti.waves_per_group;

@@ -106,6 +106,9 @@ CONFIGS = {
     "one_tier_4waves_spill": [tier(512, hash_log2=9, slack=0, waves_per_group=4)],
     "two_tiers_shape3_grid1": [tier(128, shape=3, waves_per_group=2, groups_per_cu=1),
                                tier(512, hash_log2=10, shape=0, groups_per_cu=2)],
+    # tables up to 2^15: the re-run's default (the largest tier's table x4) would not fit one
+    # wave's LDS, so it takes the largest table that does
+    "large_tables": [tier(128, hash_log2=14), tier(512, hash_log2=15)],
 }
 
 
