@@ -373,6 +373,8 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
         p.gbag = L.gbag;
         p.gbag_stride = L.bag_stride;
         p.wave_lds = (uint32_t)L.wave_lds;
+        p.vis16 = L.plan.vis16;
+        p.idbits = ix->idbits;
         // with tiers running side by side, the largest-pq tier (launched first, the longest
         // searches) gets issue priority
         p.wave_prio = (par && o == 0 && c.priority_first) ? 1 : 0;

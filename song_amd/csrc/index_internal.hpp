@@ -28,6 +28,7 @@ struct song_index {
     int device = 0;
     uint64_t n = 0;
     int dim = 0, ld = 0, metric = 0, shift = 0, gstride = 0;
+    int idbits = 32;  // ids are below 2^idbits
     float *d_data = nullptr;
     uint32_t *d_graph = nullptr;
     float *d_norms = nullptr;
@@ -55,10 +56,11 @@ namespace song {
 struct Plan {
     bool exact;
     int store, shape, hlog, slack, qcap, qld;
+    int vis16;   // compact visited table (16-bit slots)
     size_t lds;  // LDS bytes of one wave
 };
 
-size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld);
+size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16);
 // bytes of one wave slot's queue bag (qcap keys, 256-byte aligned)
 size_t bag_stride(const Plan &p);
 // shape < 0: by row width; otherwise the given lanes-per-row shape

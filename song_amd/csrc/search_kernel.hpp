@@ -88,6 +88,8 @@ struct SearchParams {
     uint32_t *work_next;
     const uint32_t *work_count;
     uint32_t wave_lds;      // LDS bytes of one wave's slice (every launch; the layout must fit)
+    int vis16;              // 1: compact visited table (16-bit slots, needs idbits - (hlog - 3) <= 13)
+    int idbits;             // ids are below 2^idbits
     unsigned char *gbag;    // queue bags, one per launch slot / persistent wave, gbag_stride bytes
     size_t gbag_stride;     // >= qcap keys
     // queries whose visited set or queue overflowed are appended here for a re-run
@@ -223,6 +225,76 @@ __device__ __forceinline__ bool vis_test(const uint32_t *vis, uint32_t mask, int
         if (v.x == 0u || v.y == 0u || v.z == 0u || v.w == 0u) return false;
         g = (g + 1u) & gmask;
     }
+}
+
+// Compact visited set (16-bit slots, half the LDS for the same slot count).  For ids below
+// 2^idbits, h = id * 0x9E3779B1 mod 2^idbits is a bijection: its top gbits = hlog - 3 bits pick
+// the home group of 8 slots (one 16-byte read), and a slot holds the other lob = idbits - gbits
+// <= 13 bits as lo + 1 (never 0) with bits 14..15 = the displacement d <= 3 of the group that
+// holds it.  Equal entries at equal displacement from equal homes mean equal h, hence equal ids,
+// so the set is exact.  An id takes the first empty slot of the first group at d = 0..3 that has
+// one (lookups stop there as in the 32-bit table); with none free in those 4 groups the insert
+// reports an overflow and the caller continues in the global table.  Lanes inserting the same id
+// read the same group state, choose the same slot and race through compare-and-swap on its word.
+__device__ __forceinline__ void vis16_key(uint32_t id, int idbits, int gbits, uint32_t &home, uint32_t &lo1) {
+    const uint32_t hm = idbits >= 32 ? 0xFFFFFFFFu : ((1u << idbits) - 1u);
+    const uint32_t h = (id * 0x9E3779B1u) & hm;
+    const int lob = idbits - gbits;
+    home = h >> lob;
+    lo1 = (h & ((1u << lob) - 1u)) + 1u;
+}
+__device__ __forceinline__ uint32_t half16(const uint4 &v, int t) {
+    const uint32_t w = (t >> 1) == 0 ? v.x : (t >> 1) == 1 ? v.y : (t >> 1) == 2 ? v.z : v.w;
+    return (t & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+// 0: present, 1: inserted, 2: no room within 4 groups
+__device__ __forceinline__ int vis16_insert(uint32_t *vis, int gbits, int idbits, uint32_t id) {
+    uint32_t home, lo1;
+    vis16_key(id, idbits, gbits, home, lo1);
+    const uint32_t gmask = (1u << gbits) - 1u;
+    for (uint32_t d = 0; d < 4u; ++d) {
+        const uint32_t g = (home + d) & gmask;
+        const uint32_t e = lo1 | (d << 14);
+        for (;;) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(vis + (g << 2));
+            bool found = false;
+            int empty = 8;
+#pragma unroll
+            for (int t = 7; t >= 0; --t) {
+                const uint32_t x = half16(v, t);
+                found |= x == e;
+                if (x == 0u) empty = t;
+            }
+            if (found) return 0;
+            if (empty == 8) break;  // group full: next displacement
+            const int wi = empty >> 1;
+            const uint32_t old = wi == 0 ? v.x : wi == 1 ? v.y : wi == 2 ? v.z : v.w;
+            const uint32_t nw = old | (e << ((empty & 1) * 16));
+            if (atomicCAS(&vis[(g << 2) + wi], old, nw) == old) return 1;
+            lds_fence();  // the word changed: read the group again
+        }
+    }
+    return 2;
+}
+__device__ __forceinline__ bool vis16_test(const uint32_t *vis, int gbits, int idbits, uint32_t id) {
+    uint32_t home, lo1;
+    vis16_key(id, idbits, gbits, home, lo1);
+    const uint32_t gmask = (1u << gbits) - 1u;
+    for (uint32_t d = 0; d < 4u; ++d) {
+        const uint32_t g = (home + d) & gmask;
+        const uint32_t e = lo1 | (d << 14);
+        const uint4 v = *reinterpret_cast<const uint4 *>(vis + (g << 2));
+        bool found = false, any_empty = false;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const uint32_t x = half16(v, t);
+            found |= x == e;
+            any_empty |= x == 0u;
+        }
+        if (found) return true;
+        if (any_empty) return false;
+    }
+    return false;
 }
 
 // insert into this query's global spill table (same scheme, device-scope CAS)
@@ -603,7 +675,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     const uint32_t vis_limit = (H >> 2) * 3u;
 
     uint32_t *vis = reinterpret_cast<uint32_t *>(smem);
-    Key *S = reinterpret_cast<Key *>(smem + (size_t)H * 4u);  // merge scratch, QH keys
+    const uint32_t vis_bytes = H * (P.vis16 ? 2u : 4u);
+    const int gbits = P.hlog - 3;                               // compact table: groups of 8
+    Key *S = reinterpret_cast<Key *>(smem + vis_bytes);        // merge scratch, QH keys
     Key *B = S + QH;
     uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));  // 256 u32
     float *qv = reinterpret_cast<float *>(cid + 256);
@@ -616,7 +690,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         return;
     }
 
-    for (uint32_t i = lane; i < (H >> 2); i += 64) reinterpret_cast<uint4 *>(vis)[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t i = lane; i < (vis_bytes >> 4); i += 64) reinterpret_cast<uint4 *>(vis)[i] = make_uint4(0u, 0u, 0u, 0u);
     for (int i = lane; i < P.qld; i += 64) qv[i] = (i < P.dim) ? P.queries[(size_t)qi * P.dim + i] : 0.0f;
     lds_fence();
 
@@ -656,7 +730,12 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             use = lane == 0;
             sid = 0u;
         }
-        const bool fresh = use && vis_insert(vis, hmask, P.hlog, sid);
+        int ins = 0;
+        if (use) ins = P.vis16 ? vis16_insert(vis, gbits, P.idbits, sid) : (vis_insert(vis, hmask, P.hlog, sid) ? 1 : 0);
+        // no room for a start vertex (up to 64 in an empty table): the query is flagged and not
+        // searched; the epilogue reports it for a re-run with a larger table
+        if (ballot(ins == 2) != 0ull) flags |= F_VIS_OVF;
+        const bool fresh = ins == 1;
         const uint64_t fm = ballot(fresh);
         const int m0 = __popcll(fm);
         if (fresh) cid[lanes_below(fm)] = sid;
@@ -785,10 +864,34 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             for (uint32_t i = lane; i < GH; i += 64) gv[i] = 0u;
             __threadfence_block();
         }
-        bool fresh = false;
+        bool fresh = false, vovf = false;
         if (lane >= 1 && lane <= deg) {
-            if (!spill) fresh = vis_insert(vis, hmask, P.hlog, r);
-            else fresh = !vis_test(vis, hmask, P.hlog, r) && gvis_insert(gv, gmask, P.ghlog, r);
+            if (!spill) {
+                if (P.vis16) {
+                    const int ins = vis16_insert(vis, gbits, P.idbits, r);
+                    fresh = ins == 1;
+                    vovf = ins == 2;
+                } else {
+                    fresh = vis_insert(vis, hmask, P.hlog, r);
+                }
+            } else {
+                const bool in_lds = P.vis16 ? vis16_test(vis, gbits, P.idbits, r) : vis_test(vis, hmask, P.hlog, r);
+                fresh = !in_lds && gvis_insert(gv, gmask, P.ghlog, r);
+            }
+        }
+        if (P.vis16 && ballot(vovf) != 0ull) {
+            // the compact table had no room near these ids: continue in the global table
+            if (!P.gvis) {
+                flags |= F_VIS_OVF;
+                return true;
+            }
+            if (!spill) {
+                spill = true;
+                gv = P.gvis + (size_t)slot_q * GH;
+                for (uint32_t i = lane; i < GH; i += 64) gv[i] = 0u;
+                __threadfence_block();
+            }
+            if (vovf) fresh = gvis_insert(gv, gmask, P.ghlog, r);
         }
         const uint64_t fm = ballot(fresh);
         const int m = __popcll(fm);
@@ -877,7 +980,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         return false;
     };
     for (;;) {
-        if (expand(std::integral_constant<int, 0>{})) break;
+        if (flags != 0u || expand(std::integral_constant<int, 0>{})) break;
         if (expand(std::integral_constant<int, 1>{})) break;
     }
 

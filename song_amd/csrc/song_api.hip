@@ -223,6 +223,7 @@ extern "C" int song_index_create(const float *data, uint64_t n, int dim, const u
     ix->metric = metric;
     ix->shift = shift;
     ix->gstride = gstride;
+    ix->idbits = n <= 1 ? 1 : 64 - __builtin_clzll((unsigned long long)(n - 1));
     auto cleanup = [&](int code) {
         song_index_destroy(ix);
         return code;
@@ -303,13 +304,14 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 
 // ---------------------------------------------------------------- launch planning
 
-// must match the per-wave layout of search_query: visited | head merge scratch (64 keys) |
-// best (pq rounded to even) | 256 candidate ids / radix bins | query (the head is in
-// registers, the bag in global memory)
-size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld) {
+// must match the per-wave layout of search_query: visited (2^hlog slots of 4 or, compact, 2
+// bytes) | head merge scratch (64 keys) | best (pq rounded to even) | 256 candidate ids / radix
+// bins | query (the head is in registers, the bag in global memory)
+size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16) {
     (void)qcap;
     size_t key = exact ? 16 : 8;
-    return ((size_t)4 << hlog) + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4;
+    return ((size_t)(vis16 ? 2 : 4) << hlog) + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 +
+           (size_t)qld * 4;
 }
 
 size_t song::bag_stride(const Plan &p) { return ((size_t)p.qcap * (p.exact ? 16 : 8) + 255) & ~(size_t)255; }
@@ -326,17 +328,20 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
     // queue buffer: pq + slack live entries, 64 of merge room, 64 of head room
     p->qcap = pq + slack + 128;
     p->qld = (ix->dim + 15) & ~15;
-    p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld);
+    // compact visited slots when an id's bits beyond the group index (groups of 8) number 0..13
+    const int lob = ix->idbits - (hlog - 3);
+    p->vis16 = (hlog >= 4 && lob >= 0 && lob <= 13) ? 1 : 0;
+    p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16);
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
     return SONG_OK;
 }
 
-// LDS visited-table size by budget, the fastest on the 1M x 128 workload: small budgets
-// gain more from waves per CU (2^10 slots; most queries continue in the global table) than
-// they lose to the global table, pq ~ 768 runs best with 2^12 slots (the ~1-3k vertices a
-// search discovers stay in LDS).  Queries whose global table overflows are re-run.
-int song::default_hlog(int pq) { return pq <= 384 ? 10 : pq > 1024 ? 13 : 12; }
+// LDS visited-table size by budget, the fastest on the 1M x 128 workload: 2^12 slots (8 KB with
+// the compact 16-bit table) keep the ~1-3k vertices a search discovers up to pq ~ 1000 in LDS
+// at 3/4 load; the few queries that discover more continue in the global table, and queries
+// whose global table overflows are re-run.
+int song::default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
 
 SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
@@ -361,6 +366,8 @@ static int launch(song_index *ix, const Plan &p, SearchParams sp, unsigned count
     sp.ghlog = kGhlog;
     sp.qld = p.qld;
     sp.wave_lds = (uint32_t)p.lds;
+    sp.vis16 = p.vis16;
+    sp.idbits = ix->idbits;
     sp.data_lp = ix->d_lp[p.store];
     sp.scale = ix->d_scale[p.store];
     sp.lp_stride = ix->lp_stride[p.store];
