@@ -77,6 +77,8 @@ typedef struct {
     int retry;     /* re-run overflowed queries with larger tables (default 1) */
     int storage;   /* SONG_STORE_* (default F32) */
     int rerank;    /* low-precision storage: exact FP32 re-rank of the pq list (default 1) */
+    int visited_bits; /* visited-table slot width: 16 (compact), 32, or 0 = compact whenever the ids
+                         fit (default); results do not depend on it */
 } song_search_opts;
 
 /* Fill opts with the defaults above. */
@@ -179,6 +181,8 @@ typedef struct {
                             (lanes x 16-byte units); -1 = by row width */
     int waves_per_group; /* queries in flight per workgroup, one per wave: 1, 2 or 4 */
     int groups_per_cu;   /* persistent grid = groups_per_cu x compute units; 0 = all that fit */
+    int visited_bits;    /* visited-table slot width: 16 (compact), 32, or 0 = compact whenever the
+                            ids fit; results do not depend on it */
 } song_tier;
 
 typedef struct {

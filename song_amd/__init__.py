@@ -36,12 +36,12 @@ class QueryParams(ctypes.Structure):
 class SearchOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slack", ctypes.c_int), ("hash_log2", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("retry", ctypes.c_int), ("storage", ctypes.c_int),
-                ("rerank", ctypes.c_int)]
+                ("rerank", ctypes.c_int), ("visited_bits", ctypes.c_int)]
 
 
 class Tier(ctypes.Structure):
     _fields_ = [("pq_max", ctypes.c_int), ("hash_log2", ctypes.c_int), ("slack", ctypes.c_int), ("shape", ctypes.c_int),
-                ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int)]
+                ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int), ("visited_bits", ctypes.c_int)]
 
 
 MAX_TIERS = 8
@@ -118,11 +118,12 @@ def metric_code(metric):
     return METRIC_NAMES[metric] if isinstance(metric, str) else int(metric)
 
 
-def make_opts(mode=MODE_AUTO, slack=32, hash_log2=0, max_steps=0, retry=1, storage=STORE_F32, rerank=1):
+def make_opts(mode=MODE_AUTO, slack=32, hash_log2=0, max_steps=0, retry=1, storage=STORE_F32, rerank=1,
+              visited_bits=0):
     o = SearchOpts()
     lib().song_search_opts_default(ctypes.byref(o))
     o.mode, o.slack, o.hash_log2, o.max_steps, o.retry = mode, slack, hash_log2, max_steps, retry
-    o.storage, o.rerank = storage, rerank
+    o.storage, o.rerank, o.visited_bits = storage, rerank, visited_bits
     return o
 
 
@@ -168,7 +169,7 @@ class Index:
         return codes, scales
 
     def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, storage=STORE_F32,
-               rerank=1):
+               rerank=1, visited_bits=0):
         """Host buffers in and out: ids int64 [nq, k] (-1 padding), dists float32, flags uint32."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
@@ -176,7 +177,7 @@ class Index:
         ids = np.empty((nq, k), dtype=np.uint64)
         ds = np.empty((nq, k), dtype=np.float32)
         fl = np.empty(nq, dtype=np.uint32)
-        o = make_opts(mode, slack, hash_log2, 0, retry, storage, rerank)
+        o = make_opts(mode, slack, hash_log2, 0, retry, storage, rerank, visited_bits)
         _check(lib().song_search(self._h, _ptr(q), nq, pq, k, _ptr(ids), _ptr(ds), _ptr(fl), ctypes.byref(o)))
         out = ids.astype(np.int64)
         out[ids == np.uint64(0xFFFFFFFFFFFFFFFF)] = -1
@@ -184,7 +185,7 @@ class Index:
 
     def search_device(self, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
                       mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, stream=None, storage=STORE_F32, rerank=1,
-                      max_steps=0, entry=None, steps=None, subset=None):
+                      max_steps=0, entry=None, steps=None, subset=None, visited_bits=0):
         """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k].
 
         entry: int32 [nq] or [nq, E] start vertices (learned form; default vertex 0);
@@ -196,7 +197,7 @@ class Index:
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
-        o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank)
+        o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank, visited_bits)
         qp = QueryParams()
         qp.pq, qp.k = (pq_arr.data_ptr() if pq_arr is not None else None), (k_arr.data_ptr() if k_arr is not None else None)
         qp.entry = entry.data_ptr() if entry is not None else None
@@ -233,13 +234,13 @@ def resolve_mode(index, queries, stream=None):
     return m.value
 
 
-TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu")
+TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu", "visited_bits")
 
 
-def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0):
+def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0, visited_bits=0):
     """One tier of a batch configuration (see song_tier in include/song_amd.h)."""
     return dict(pq_max=pq_max, hash_log2=hash_log2, slack=slack, shape=shape, waves_per_group=waves_per_group,
-                groups_per_cu=groups_per_cu)
+                groups_per_cu=groups_per_cu, visited_bits=visited_bits)
 
 
 class Batch:

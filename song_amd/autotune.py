@@ -9,6 +9,8 @@ coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
   shape            lanes per candidate row (the sub-wave width of one distance)
   slack            heap depth beyond pq (tie room; overflows cost a re-run)
   groups_per_cu    persistent grid size (0 = all that fit)
+  visited_bits     visited-table slot width (0 = compact 16-bit slots where the ids fit, 32 = 32-bit:
+                   fewer instructions per probe, twice the LDS per slot)
 
 Each candidate runs the tier's queries through song_batch_run and is timed with events
 (median of `reps`); a candidate is accepted only if every result equals the default
@@ -106,7 +108,8 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                                  ("hash_log2", tuple(range(8, 15))),
                                  ("shape", SHAPES),
                                  ("slack", (0, 8, 32, 128)),
-                                 ("groups_per_cu", (0, 1, 2, 4, 8))):
+                                 ("groups_per_cu", (0, 1, 2, 4, 8)),
+                                 ("visited_bits", (0, 32))):
                 cur = best[key]
                 if key == "hash_log2" and cur == 0:
                     cur = -1

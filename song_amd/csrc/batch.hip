@@ -127,6 +127,7 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     t.shape = -1;
     t.waves_per_group = 1;
     t.groups_per_cu = 0;
+    t.visited_bits = 0;
     c->tier[0] = t;
     c->retry = t;
     c->retry.hash_log2 = 0;  // the largest tier's filter x 4
@@ -159,6 +160,8 @@ static int check_tier(const song_tier &t, int idx) {
         return fail(SONG_E_ARG, "tier %d: waves_per_group=%d not 1, 2 or 4", idx, t.waves_per_group);
     if (t.groups_per_cu < 0 || t.groups_per_cu > 64)
         return fail(SONG_E_ARG, "tier %d: groups_per_cu=%d", idx, t.groups_per_cu);
+    if (t.visited_bits != 0 && t.visited_bits != 16 && t.visited_bits != 32)
+        return fail(SONG_E_ARG, "tier %d: visited_bits=%d not 0, 16 or 32", idx, t.visited_bits);
     return SONG_OK;
 }
 
@@ -221,12 +224,12 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         if (!is_retry) top_hlog = std::max(top_hlog, hlog);
         Launch L;
         L.pq_max = ti.pq_max;
-        int rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape);
+        int rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
         // the re-run's default table (the largest tier's, 4x) may not fit one wave's LDS: take the
         // largest that does (the re-run also has a 4x larger global spill table)
         while (rc == SONG_E_OVERFLOW && is_retry && !ti.hash_log2 && hlog > top_hlog) {
             --hlog;
-            rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape);
+            rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
         }
         if (rc != SONG_OK) return cleanup(rc);
         L.fn = search_fn(ix->metric, L.plan, true);

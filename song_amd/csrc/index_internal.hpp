@@ -64,7 +64,9 @@ size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16);
 // bytes of one wave slot's queue bag (qcap keys, 256-byte aligned)
 size_t bag_stride(const Plan &p);
 // shape < 0: by row width; otherwise the given lanes-per-row shape
-int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape = -1);
+// visited_bits: 16 or 32 = slot width (16 only where the ids fit, else 32); 0 = 16 where they fit
+int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape = -1,
+              int visited_bits = 0);
 int default_hlog(int pq);
 SearchFn search_fn(int metric, const Plan &p, bool persistent = false);
 // integer test and range of a device float array (synchronizes st)

@@ -84,6 +84,7 @@ extern "C" void song_search_opts_default(song_search_opts *o) {
     o->retry = 1;
     o->storage = SONG_STORE_F32;
     o->rerank = 1;
+    o->visited_bits = 0;
 }
 
 // Low-precision copy of one row per thread: BF16 with a power-of-two scale (round to
@@ -316,7 +317,8 @@ size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis1
 
 size_t song::bag_stride(const Plan &p) { return ((size_t)p.qcap * (p.exact ? 16 : 8) + 255) & ~(size_t)255; }
 
-int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape) {
+int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape,
+                    int visited_bits) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
     if (store != SONG_STORE_F32 && !ix->d_lp[store]) return fail(SONG_E_ARG, "storage %d not built", store);
     p->exact = exact && store == SONG_STORE_F32;
@@ -330,7 +332,7 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
     p->qld = (ix->dim + 15) & ~15;
     // compact visited slots when an id's bits beyond the group index (groups of 8) number 0..13
     const int lob = ix->idbits - (hlog - 3);
-    p->vis16 = (hlog >= 4 && lob >= 0 && lob <= 13) ? 1 : 0;
+    p->vis16 = (visited_bits != 32 && hlog >= 4 && lob >= 0 && lob <= 13) ? 1 : 0;
     p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16);
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
@@ -434,6 +436,8 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
 
     if (opts.storage < SONG_STORE_F32 || opts.storage > SONG_STORE_FP8)
         return fail(SONG_E_ARG, "storage=%d", opts.storage);
+    if (opts.visited_bits != 0 && opts.visited_bits != 16 && opts.visited_bits != 32)
+        return fail(SONG_E_ARG, "visited_bits=%d not 0, 16 or 32", opts.visited_bits);
     bool exact = true;
     if (opts.storage != SONG_STORE_F32 || opts.mode == SONG_MODE_FP32) {
         exact = false;
@@ -451,7 +455,7 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
     int hlog = opts.hash_log2 > 0 ? opts.hash_log2 : default_hlog(pq);
     hlog = std::max(8, std::min(hlog, 16));
     Plan plan;
-    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan);
+    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan, -1, opts.visited_bits);
     if (rc != SONG_OK) return rc;
 
     uint32_t *flags = d_flags;
@@ -524,7 +528,7 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
             Plan p2;
             int h2 = vis ? plan.hlog + 2 : plan.hlog;
             int s2 = que ? plan.slack * 4 + 64 : plan.slack;
-            rc = make_plan(ix, exact, opts.storage, pq, s2, std::min(h2, 16), &p2);
+            rc = make_plan(ix, exact, opts.storage, pq, s2, std::min(h2, 16), &p2, -1, opts.visited_bits);
             if (rc != SONG_OK) break;
             plan = p2;
             if (!d_idx && hipMalloc(&d_idx, nq * sizeof(uint32_t)) != hipSuccess) {

@@ -91,3 +91,15 @@ def test_small_tables_force_spill_and_retry_and_stay_exact():
     # still equal the reference's
     ids, _, flags = ix.search(query, meta["pq"], mode=song_amd.MODE_EXACT, slack=0, hash_log2=9)
     assert np.array_equal(ids, ref_ids)
+
+
+@pytest.mark.parametrize("hash_log2", [10, 12, 14])
+def test_visited_slot_widths_give_the_reference_ids(hash_log2):
+    # compact 16-bit and 32-bit visited slots, with and without spills: the same exact ids
+    meta, base, query, ref_ids, graph = golden_cases.load("tie_l2_3k")
+    ix = song_amd.Index(base, graph, meta["metric"])
+    print(f"[visited] tie_l2_3k hash_log2 {hash_log2}", flush=True)
+    for bits in (16, 32):
+        ids, ds, flags = ix.search(query, meta["pq"], mode=song_amd.MODE_EXACT, hash_log2=hash_log2,
+                                   visited_bits=bits)
+        assert np.array_equal(ids, ref_ids), bits

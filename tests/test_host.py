@@ -122,3 +122,36 @@ def test_index_create_validates_graph_without_gpu_side_effects():
     g[1, 1] = 9  # id beyond n
     with pytest.raises(song_amd.SongError, match="neighbour id"):
         song_amd.Index(base, g, "l2")
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """The Python mirrors of song_search_opts, song_tier and song_batch_config have the sizes and
+    field offsets the C compiler gives include/song_amd.h (a mismatch corrupts every call)."""
+    import ctypes
+    import shutil
+    import subprocess
+
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not installed")
+    fields = {"song_search_opts": (song_amd.SearchOpts, ["mode", "hash_log2", "rerank", "visited_bits"]),
+              "song_tier": (song_amd.Tier, ["pq_max", "groups_per_cu", "visited_bits"]),
+              "song_batch_config": (song_amd.BatchConfig, ["tier", "retry", "parallel_tiers", "priority_first"])}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "song_amd.h"', "int main(void) {"]
+    for cname, (_, names) in fields.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in names:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n"):
+        if line:
+            cname, what, val = line.split()
+            got[(cname, what)] = int(val)
+    for cname, (cls, names) in fields.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f in names:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
