@@ -174,7 +174,7 @@ enum { SONG_FLAG_BAD_PARAMS = 8u }; /* batch: pq outside every tier (query not s
 
 typedef struct {
     int pq_max;          /* tier of the queries with pq in (previous pq_max, pq_max] */
-    int hash_log2;       /* visited filter: 2^hash_log2 LDS slots per wave (8..16), spilling to a
+    int hash_log2;       /* visited table: 2^hash_log2 LDS slots per wave (8..16), spilling to a
                             global table; 0 = sized from pq_max */
     int slack;           /* heap depth: queue entries kept beyond pq for distance ties (>= 0) */
     int shape;           /* lanes per candidate row: 0..4 = 4x2, 4x8, 4x16, 16x16, 64x16
