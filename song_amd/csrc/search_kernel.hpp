@@ -18,7 +18,8 @@
 // (the drops above are applied when it nears its capacity).
 //
 // Per-wave LDS layout (bytes, in order):
-//   visited  : H x u32 open-addressing set of (id + 1), probed in groups of 4, no deletion
+//   visited  : H slots, no deletion: compact 16-bit entries in groups of 8 (vis16_insert) or
+//              32-bit (id + 1) in groups of 4 (vis_insert), one 16-byte read per group
 //   scratch  : 64 keys    merge staging of the queue head (the head itself is one key per
 //                          lane in registers, ascending over lanes [h, h + nq), nq <= 64)
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
