@@ -159,6 +159,24 @@ __device__ __forceinline__ bool lt(const KeyD &a, const KeyD &b) {
 __device__ __forceinline__ KeyF rl_key(const KeyF &k, int lane) { return k.rl(lane); }
 __device__ __forceinline__ KeyD rl_key(const KeyD &k, int lane) { return k.rl(lane); }
 
+// whole-wave lane shifts by one (DPP wave_shl:1 / wave_shr:1): shl1 gives lane i the value of
+// lane i + 1 (lane 63 keeps its own), shr1 the value of lane i - 1 (lane 0 keeps its own)
+template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, 0xF, 0xF, false);
+}
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
+    return ((uint64_t)dpp32<CTRL>((uint32_t)(x >> 32)) << 32) | dpp32<CTRL>((uint32_t)x);
+}
+__device__ __forceinline__ KeyF shl1(const KeyF &k) { return KeyF{dpp64<DPP_WAVE_SHL1>(k.v)}; }
+__device__ __forceinline__ KeyF shr1(const KeyF &k) { return KeyF{dpp64<DPP_WAVE_SHR1>(k.v)}; }
+__device__ __forceinline__ KeyD shl1(const KeyD &k) {
+    return KeyD{dpp64<DPP_WAVE_SHL1>(k.d), dpp32<DPP_WAVE_SHL1>(k.i), 0u};
+}
+__device__ __forceinline__ KeyD shr1(const KeyD &k) {
+    return KeyD{dpp64<DPP_WAVE_SHR1>(k.d), dpp32<DPP_WAVE_SHR1>(k.i), 0u};
+}
+
 template <bool EXACT> struct KeySel { using T = KeyF; };
 template <> struct KeySel<true> { using T = KeyD; };
 
@@ -244,9 +262,14 @@ __device__ __forceinline__ void vis16_key(uint32_t id, int idbits, int gbits, ui
     home = h >> lob;
     lo1 = (h & ((1u << lob) - 1u)) + 1u;
 }
-__device__ __forceinline__ uint32_t half16(const uint4 &v, int t) {
-    const uint32_t w = (t >> 1) == 0 ? v.x : (t >> 1) == 1 ? v.y : (t >> 1) == 2 ? v.z : v.w;
-    return (t & 1) ? (w >> 16) : (w & 0xFFFFu);
+// the zero 16-bit halves of x, flagged in bits 15 and 31: (x - 0x00010001) & ~x & 0x80008000.
+// A zero low half is always flagged and a nonzero one never is; the high half's flag is exact
+// unless the low half is zero (its borrow can flag a high half of 1).  So "some half is zero" and
+// "the lowest zero half" read exactly from the flags.
+__device__ __forceinline__ uint32_t zero_halves(uint32_t x) { return (x - 0x00010001u) & ~x & 0x80008000u; }
+// some half of the group equals e (ee = e in both halves)
+__device__ __forceinline__ bool group_has(const uint4 &v, uint32_t ee) {
+    return (zero_halves(v.x ^ ee) | zero_halves(v.y ^ ee) | zero_halves(v.z ^ ee) | zero_halves(v.w ^ ee)) != 0u;
 }
 // 0: present, 1: inserted, 2: no room within 4 groups
 __device__ __forceinline__ int vis16_insert(uint32_t *vis, int gbits, int idbits, uint32_t id) {
@@ -258,19 +281,14 @@ __device__ __forceinline__ int vis16_insert(uint32_t *vis, int gbits, int idbits
         const uint32_t e = lo1 | (d << 14);
         for (;;) {
             const uint4 v = *reinterpret_cast<const uint4 *>(vis + (g << 2));
-            bool found = false;
-            int empty = 8;
-#pragma unroll
-            for (int t = 7; t >= 0; --t) {
-                const uint32_t x = half16(v, t);
-                found |= x == e;
-                if (x == 0u) empty = t;
-            }
-            if (found) return 0;
-            if (empty == 8) break;  // group full: next displacement
-            const int wi = empty >> 1;
-            const uint32_t old = wi == 0 ? v.x : wi == 1 ? v.y : wi == 2 ? v.z : v.w;
-            const uint32_t nw = old | (e << ((empty & 1) * 16));
+            if (group_has(v, e * 0x00010001u)) return 0;
+            // the first empty slot: the first word with a zero half, then its lower zero half
+            uint32_t old = v.x, z = zero_halves(v.x), wi = 0u;
+            if (z == 0u) { old = v.y; z = zero_halves(v.y); wi = 1u; }
+            if (z == 0u) { old = v.z; z = zero_halves(v.z); wi = 2u; }
+            if (z == 0u) { old = v.w; z = zero_halves(v.w); wi = 3u; }
+            if (z == 0u) break;  // group full: next displacement
+            const uint32_t nw = old | (e << ((z & 0x8000u) ? 0 : 16));
             if (atomicCAS(&vis[(g << 2) + wi], old, nw) == old) return 1;
             lds_fence();  // the word changed: read the group again
         }
@@ -285,15 +303,8 @@ __device__ __forceinline__ bool vis16_test(const uint32_t *vis, int gbits, int i
         const uint32_t g = (home + d) & gmask;
         const uint32_t e = lo1 | (d << 14);
         const uint4 v = *reinterpret_cast<const uint4 *>(vis + (g << 2));
-        bool found = false, any_empty = false;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const uint32_t x = half16(v, t);
-            found |= x == e;
-            any_empty |= x == 0u;
-        }
-        if (found) return true;
-        if (any_empty) return false;
+        if (group_has(v, e * 0x00010001u)) return true;
+        if ((zero_halves(v.x) | zero_halves(v.y) | zero_halves(v.z) | zero_halves(v.w)) != 0u) return false;
     }
     return false;
 }
@@ -527,6 +538,19 @@ __device__ __forceinline__ int prefix_count(const Key *arr, int n, Pred below) {
     return lo + __popcll(ballot(v && below(e)));
 }
 
+// arr[p, e) moves to arr[p + 1, e + 1) and arr[p] = x (blocks of 64, last block first, so each
+// block is read before the block below it writes into its first slot)
+template <class Key> __device__ __forceinline__ void shift_insert(Key *arr, int p, int e, const Key &x) {
+    const int lane = lane_id();
+    for (int blk = (e - 1) >> 6; blk >= (p >> 6) && e > p; --blk) {
+        const int i = blk * 64 + lane;
+        const bool live = i >= p && i < e;
+        const Key v = arr[live ? i : 0];
+        if (live) arr[i + 1] = v;
+    }
+    if (lane == 0) arr[p] = x;
+}
+
 // ---------------------------------------------------------------- wave-wide key sets
 
 __device__ __forceinline__ KeyF shfl_key(const KeyF &k, int src) {
@@ -717,6 +741,10 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     const uint32_t GH = 1u << P.ghlog, gmask = GH - 1u, glimit = (GH >> 2) * 3u;
     int h = 0, nq = 0, nbest = 0;
     uint64_t bmax = ~0ull;  // distance key of best[pq-1] once the list is full
+    // the best list's last positions [tb, tb + 64) live in registers: lane j holds position
+    // tb + j (+inf past nbest); the front [0, tb) is in LDS.  tb = max(0, nbest - 64).
+    Key bt = Key::inf();
+    int tb = 0;
     // the queue head lives in registers: lanes [h, h + nq) hold its keys in ascending order
     Key hq = Key::inf();
 
@@ -816,37 +844,44 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         pid[Y] = nq > 0 ? hq.rl(h).id() : NONE;
         prow[Y] = adj_row(nq > 0 ? pid[Y] : 0u);
 
-        // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195):
-        // find its position, shift the entries behind it one place (last block first)
+        // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195).
+        // Expansions come out in nearly ascending order, so `now` almost always lands in the
+        // register tail: one compare and a lane shift, no LDS round trip.
         {
-            // the list's last block [lb, nbest): `now` usually lands there (expansions come
-            // out in nearly ascending order), so one read gives the position and the entries
-            // to shift; otherwise search [0, lb)
-            const int lb = nbest > 64 ? nbest - 64 : 0;
-            const int i0 = lb + lane;
-            const bool v0 = i0 < nbest;
-            const Key e0 = v0 ? B[i0] : Key::inf();
-            const uint64_t below0 = ballot(v0 && lt(e0, now));
-            const bool in_last = lb == 0 || (below0 & 1ull);
-            const int p = in_last ? lb + __popcll(below0)
-                                  : prefix_count(B, lb, [&](const Key &e) { return lt(e, now); });
+            const uint64_t below = ballot(lt(bt, now));
+            const bool in_tail = tb == 0 || (below & 1ull);
+            const int p = in_tail ? tb + __popcll(below)
+                                  : prefix_count(B, tb, [&](const Key &e) { return lt(e, now); });
             if (p < k && lane == 0) settle = steps;
             if (p < pq) {
-                const int last = min(nbest, pq - 1);  // entries [p, last) move to [p+1, last+1)
-                if (in_last) {
-                    if (v0 && i0 >= p && i0 < last) B[i0 + 1] = e0;
-                } else {
-                    for (int blk = (last - 1) >> 6; blk >= (p >> 6) && last > p; --blk) {
-                        const int i = blk * 64 + lane;
-                        const bool live = i >= p && i < last;
-                        const Key e = B[live ? i : 0];
-                        if (live) B[i + 1] = e;
+                const int j = p - tb;  // tail lane of the new entry; negative: it lands in the front
+                if (nbest >= 64 && nbest < pq) {
+                    // growing past 64 entries: tb advances by one and position tb joins the front
+                    if (j >= 0) {
+                        // the 65 entries [tb, nbest] with `now` at j: the first goes to the front,
+                        // lanes below j - 1 take their upper neighbour's entry
+                        if (lane == 0) B[tb] = (j == 0) ? now : bt;
+                        const Key up = shl1(bt);
+                        bt = (lane < j - 1) ? up : (lane == j - 1) ? now : bt;
+                    } else {
+                        // front [p, tb) moves to [p + 1, tb + 1); the tail keeps its entries
+                        shift_insert(B, p, tb, now);
                     }
+                    ++tb;
+                } else if (j >= 0) {
+                    // below 64 entries (tb = 0), or full (the entry at pq - 1 leaves)
+                    const Key dn = shr1(bt);
+                    bt = (lane < j) ? bt : (lane == j) ? now : dn;
+                } else {
+                    // full, `now` in the front: the front's last entry moves into the tail
+                    const Key fl = B[tb - 1];
+                    shift_insert(B, p, tb - 1, now);
+                    const Key dn = shr1(bt);
+                    bt = (lane == 0) ? fl : dn;
                 }
-                if (lane == 0) B[p] = now;
-                // the new largest entry: `now` itself, or the one that was at pq - 2 (still in e0)
-                if (nbest + 1 >= pq) bmax = (p == pq - 1) ? now.dk() : e0.rl(pq - 2 - lb).dk();
                 nbest = min(nbest + 1, pq);
+                if (lane >= nbest - tb) bt = Key::inf();
+                if (nbest == pq) bmax = bt.rl(pq - 1 - tb).dk();
                 lds_fence();
             }
         }
@@ -984,6 +1019,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (flags != 0u || expand(std::integral_constant<int, 0>{})) break;
         if (expand(std::integral_constant<int, 1>{})) break;
     }
+    // the whole best list in LDS from here on
+    if (lane < nbest - tb) B[tb + lane] = bt;
+    lds_fence();
 
     const int cnt = min(k, nbest);
     const size_t ob = (size_t)qi * (size_t)P.out_ld;
