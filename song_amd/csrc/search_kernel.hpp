@@ -168,6 +168,12 @@ constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
 template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
     return ((uint64_t)dpp32<CTRL>((uint32_t)(x >> 32)) << 32) | dpp32<CTRL>((uint32_t)x);
 }
+// c ? a : b field by field (a conditional over two key lvalues selects between their addresses,
+// which puts the keys in scratch memory)
+__device__ __forceinline__ KeyF ksel(bool c, const KeyF &a, const KeyF &b) { return KeyF{c ? a.v : b.v}; }
+__device__ __forceinline__ KeyD ksel(bool c, const KeyD &a, const KeyD &b) {
+    return KeyD{c ? a.d : b.d, c ? a.i : b.i, 0u};
+}
 __device__ __forceinline__ KeyF shl1(const KeyF &k) { return KeyF{dpp64<DPP_WAVE_SHL1>(k.v)}; }
 __device__ __forceinline__ KeyF shr1(const KeyF &k) { return KeyF{dpp64<DPP_WAVE_SHR1>(k.v)}; }
 __device__ __forceinline__ KeyD shl1(const KeyD &k) {
@@ -577,19 +583,36 @@ __device__ __forceinline__ Key bitonic_steps(Key v, int K, int J) {
 }
 
 // The c = min(64, n) smallest keys of arr[0..n) (any order), ascending in lanes 0..c-1 with
-// +inf above.  Each 64-key block is sorted (bitonic) and merged into the running best 64:
-// min(best[i], block[63 - i]) holds the 64 smallest of both as a bitonic sequence.
+// +inf above.  The first 64-key block is sorted (bitonic); each further block is compared with the
+// running 64th smallest.  A block with no key below it is skipped, up to SEL_INSERT keys below it
+// are inserted one at a time (rank by ballot, the lanes above shift up one), more are sorted and
+// merged: min(best[i], block[63 - i]) holds the 64 smallest of both as a bitonic sequence.
+// Bags are appended in nearly ascending order, so most blocks after the first are skipped or
+// take a few insertions.
+constexpr int SEL_INSERT = 16;
 template <class Key>
 __device__ __forceinline__ Key select64(const Key *arr, int n) {
     const int lane = lane_id();
-    Key best = Key::inf();
-    Key nxt = (lane < n) ? arr[lane] : Key::inf();
-    for (int b0 = 0; b0 < n; b0 += 64) {
+    Key best = (lane < n) ? arr[lane] : Key::inf();
+    // the next block's load is in flight while this one is sorted or scanned
+    Key nxt = (64 + lane < n) ? arr[64 + lane] : Key::inf();
+    for (int k = 2; k <= 64; k <<= 1) best = bitonic_steps(best, k, k);
+    for (int b0 = 64; b0 < n; b0 += 64) {
         Key v = nxt;
-        // the next block's load is in flight while this one is sorted
         if (b0 + 64 < n) nxt = (b0 + 64 + lane < n) ? arr[b0 + 64 + lane] : Key::inf();
-        for (int k = 2; k <= 64; k <<= 1) v = bitonic_steps(v, k, k);
-        best = bitonic_steps(kmin(best, shfl_key(v, 63 - lane)), 64, 64);
+        uint64_t m = ballot(lt(v, best.rl(63)));
+        if (__popcll(m) > SEL_INSERT) {
+            for (int k = 2; k <= 64; k <<= 1) v = bitonic_steps(v, k, k);
+            best = bitonic_steps(kmin(best, shfl_key(v, 63 - lane)), 64, 64);
+        } else {
+            for (; m != 0ull; m &= m - 1ull) {
+                const Key x = v.rl(__builtin_ctzll(m));
+                // rank 64 (x above the current 64th, lowered by earlier insertions): no change
+                const int r = __popcll(ballot(lt(best, x)));
+                const Key dn = shr1(best);
+                best = ksel(lane < r, best, ksel(lane == r, x, dn));
+            }
+        }
     }
     return best;
 }
@@ -860,9 +883,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                     if (j >= 0) {
                         // the 65 entries [tb, nbest] with `now` at j: the first goes to the front,
                         // lanes below j - 1 take their upper neighbour's entry
-                        if (lane == 0) B[tb] = (j == 0) ? now : bt;
+                        if (lane == 0) B[tb] = ksel(j == 0, now, bt);
                         const Key up = shl1(bt);
-                        bt = (lane < j - 1) ? up : (lane == j - 1) ? now : bt;
+                        bt = ksel(lane < j - 1, up, ksel(lane == j - 1, now, bt));
                     } else {
                         // front [p, tb) moves to [p + 1, tb + 1); the tail keeps its entries
                         shift_insert(B, p, tb, now);
@@ -871,16 +894,16 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                 } else if (j >= 0) {
                     // below 64 entries (tb = 0), or full (the entry at pq - 1 leaves)
                     const Key dn = shr1(bt);
-                    bt = (lane < j) ? bt : (lane == j) ? now : dn;
+                    bt = ksel(lane < j, bt, ksel(lane == j, now, dn));
                 } else {
                     // full, `now` in the front: the front's last entry moves into the tail
                     const Key fl = B[tb - 1];
                     shift_insert(B, p, tb - 1, now);
                     const Key dn = shr1(bt);
-                    bt = (lane == 0) ? fl : dn;
+                    bt = ksel(lane == 0, fl, dn);
                 }
                 nbest = min(nbest + 1, pq);
-                if (lane >= nbest - tb) bt = Key::inf();
+                bt = ksel(lane >= nbest - tb, Key::inf(), bt);
                 if (nbest == pq) bmax = bt.rl(pq - 1 - tb).dk();
                 lds_fence();
             }

@@ -8,6 +8,10 @@ follows it lane by lane -- the DPP shifts (shl1: lane i takes lane i + 1, shr1: 
 lane i - 1, the edge lane keeps its own), the entry handed to the front while the list grows
 past 64, the front's last entry handed back to the tail when a full list takes a key in its
 front -- and checks every step against a sorted list.
+
+The head refill's select64 (the 64 smallest keys of an unsorted bag, ascending in lanes 0..63)
+is modelled the same way: bitonic sorts and merges as sorting, the one-at-a-time insertion of
+the few keys of a block below the running 64th smallest lane by lane.
 """
 import numpy as np
 import pytest
@@ -115,3 +119,69 @@ def test_swar_group_test_matches_direct_compares():
                 break
         first_empty = next((t for t in range(8) if halves[t] == 0), None)
         assert slot == first_empty
+
+
+SEL_INSERT = 16
+
+
+def select64(arr):
+    """Lane-level model of the kernel's select64 over arr (list of distinct keys)."""
+    n = len(arr)
+    blk = lambda b0: [arr[b0 + i] if b0 + i < n else INF for i in range(64)]  # noqa: E731
+    best = sorted(blk(0))
+    paths = {"skip": 0, "insert": 0, "merge": 0}
+    for b0 in range(64, n, 64):
+        v = blk(b0)
+        top = best[63]
+        m = [i for i in range(64) if v[i] < top]
+        if len(m) > SEL_INSERT:
+            best = sorted(best + sorted(v))[:64]  # bitonic sort of v, then the bitonic merge
+            paths["merge"] += 1
+            continue
+        paths["insert" if m else "skip"] += 1
+        for j in m:
+            x = v[j]
+            r = sum(e < x for e in best)
+            dn = shr1(best)
+            best = [best[i] if i < r else x if i == r else dn[i] for i in range(64)]
+    return best, paths
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 127, 128, 129, 500, 1100])
+@pytest.mark.parametrize("order", ["random", "ascending_noise", "descending", "ties"])
+def test_select64_returns_the_64_smallest_in_order(n, order):
+    rng = np.random.default_rng(n * 3 + len(order))
+    if order == "random":
+        d = rng.random(n)
+    elif order == "ascending_noise":
+        d = np.sort(rng.random(n)) + rng.normal(0, 0.05, n)
+    elif order == "descending":
+        d = -np.arange(n, dtype=np.float64)
+    else:
+        d = rng.integers(0, 5, n).astype(np.float64)  # few distances: order by id
+    ids = rng.permutation(n)
+    keys = [(float(x), int(i)) for x, i in zip(d, ids)]
+    best, paths = select64(keys)
+    ref = sorted(keys)[:64]
+    assert best == ref + [INF] * (64 - len(ref))
+    if order == "ascending_noise" and n >= 500:
+        assert paths["merge"] <= 2  # nearly ascending bags rarely need a full merge
+    if order == "descending":
+        # every key beats the running 64th: blocks with more than SEL_INSERT keys are merged
+        assert paths["merge"] == sum(min(64, n - b0) > SEL_INSERT for b0 in range(64, n, 64))
+        assert paths["skip"] == 0
+
+
+def test_select64_takes_keys_just_below_the_running_64th():
+    # every later block holds keys between the running 63rd and 64th smallest (and above), so a
+    # threshold one rank off, or an insertion at the wrong rank, changes the result
+    keys = [(float(2 * i), i) for i in range(64)]
+    nid = 64
+    for b in range(6):
+        top = sorted(keys)[63][0]
+        blk = [(top - 0.5 - 0.001 * j, nid + j) for j in range(3)] + [(top + 1 + j, nid + 3 + j) for j in range(61)]
+        nid += 64
+        keys += blk
+    best, paths = select64(keys)
+    assert best == sorted(keys)[:64]
+    assert paths["insert"] == 6
