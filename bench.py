@@ -262,6 +262,12 @@ def bench_c2(args, torch, song_amd):
         fit = ls.fit(train, pq_label=256, k=k, epochs=8, batch=1024, lr=2e-3, verbose=False)
         log(f"learned predictor trained in {time.time() - t2:.1f}s: top1 acc {fit['train_top1_acc']:.3f} "
             f"mean conf {fit['mean_conf']:.3f} mean settle {fit['mean_settle']:.1f}")
+        fcal, fcap = None, None
+        if args.fallback_quantile > 0:
+            fcal = ls.calibrate_fallback_cap(train, pq, k=k, quantile=args.fallback_quantile)
+            fcap = fcal["cap"]
+            log(f"fallback cap {fcap} expansions (settle quantile {args.fallback_quantile} over {fcal['queries']} "
+                f"training queries, settle mean {fcal['settle_mean']}, max {fcal['settle_max']})")
         lst = torch.empty((nq, 8), dtype=torch.int32, device=dev)
         lsweep = []
         best = None
@@ -269,7 +275,7 @@ def bench_c2(args, torch, song_amd):
             for sc in [float(x) for x in args.step_scales.split(",")]:
                 scale = sc if sc > 0 else 1e6
                 lb = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                          storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32)
+                                          storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap)
                 plan = lb.run(q, stats=lst)
                 torch.cuda.synchronize()
                 rl = recall(lb.ids)
@@ -289,7 +295,7 @@ def bench_c2(args, torch, song_amd):
             ent, lpq, scale, stl = best
             tt = time.time()
             lb0 = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                       storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32)
+                                       storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap)
             p = lb0.run(q)
             table = autotune.tune(ix, q, p[0], lb0.k_arr, tiers=lb0.tiers, mode=song_amd.MODE_FP32,
                                   storage=song_amd.STORE_BF16, reps=3, max_queries_per_tier=nq, k_ld=k, entry=p[1],
@@ -299,11 +305,20 @@ def bench_c2(args, torch, song_amd):
             lb0.batch.close()
             log(f"learned tiers autotuned in {time.time() - tt:.1f}s: {table['tiers']}")
             lb = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                      storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, tiers=table["tiers"])
-            lb.capture(q)
+                                      storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, tiers=table["tiers"],
+                                      fallback_cap=fcap)
+            plan_g = lb.capture(q)
             lb.replay()
             torch.cuda.synchronize()
             same = bool(torch.equal(lb.ids, eager_ids))
+            # the fallback queries against the uncapped plain search of the same queries
+            fbm = ~plan_g[3]
+            n_fb = int(fbm.sum().item())
+            fb_capped = int(((lb.flags[fbm] & song_amd.FLAG_STEP_CAP) != 0).sum().item())
+            fb_equal = int((lb.ids[fbm][:, :k] == plain_ids[fbm][:, :k]).all(1).sum().item())
+            fallback_info = {"queries": n_fb, "cap": fcap, "capped": fb_capped, "top10_ids_equal_plain": fb_equal,
+                             "calibration": fcal}
+            log(f"fallback: {n_fb} queries, cap {fcap}: {fb_capped} capped, {fb_equal} with the plain search's top-{k}")
             r_final = recall(lb.ids)
             ms_l, wall_l = timed_steps(lb.replay, args.steps, args.warmup, torch, stream)
             alg_l = alg_bytes(stl, dim, dim * 2 + 4, nq)
@@ -312,7 +327,7 @@ def bench_c2(args, torch, song_amd):
             learned_info = dict(ent, qps=round(nq / (ms_l / 1e3), 1), ms_per_step=round(ms_l, 4),
                                 wall_ms_per_step=round(wall_l, 4), recall_graph=round(r_final, 4),
                                 graph_ids_equal_eager=same, algorithmic_bytes_per_step=alg_l, tiers=table["tiers"],
-                                training=fit_out)
+                                fallback=fallback_info, training=fit_out)
             log(f"learned+bf16 batched form: {learned_info['qps']:.0f} q/s ({learned_info['qps'] / qps_plain:.2f}x "
                 f"plain), recall {r_final:.4f}, graph == eager: {same}")
         learned_info = learned_info or {"error": "no learned setting reached the target recall"}
@@ -559,6 +574,9 @@ def main():
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--learned-pqs", default="64,96,128,192,256")
     ap.add_argument("--step-scales", default="1.5,2.5,0")
+    ap.add_argument("--fallback-quantile", type=float, default=0.999,
+                    help="fallback expansion cap = this quantile of the plain search's top-k settle step over "
+                         "low-confidence training queries (<= 0: uncapped fallback)")
     ap.add_argument("--no-recall", action="store_true",
                     help="with --pq: skip the ground truth and recall (profiling runs)")
     ap.add_argument("--copy-mb", type=int, default=0,

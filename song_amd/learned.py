@@ -18,7 +18,12 @@ Pipeline (all heavy work on the GPU):
            it and its 3 nearest pivots -- at least the threshold) start at the
            neighbourhood's pivot vertices with budget pq and an expansion cap
            from the stop head; the rest fall back to the plain full-budget
-           search from vertex 0.  LearnedSearch.search runs the two groups as
+           search from vertex 0, optionally under one expansion cap calibrated
+           on training queries (calibrate_fallback_cap: a high quantile of the
+           step at which the plain search's top-k settles), which cuts the
+           latency tail of searches that keep expanding long after their top-k
+           is final and leaves every query that settles within it with exactly
+           the plain search's top-k.  LearnedSearch.search runs the two groups as
            two calls of song_search_device_ex with query subsets, on two
            streams; LearnedBatch turns the split into per-query (pq, entries,
            cap) arrays on the GPU and runs every query in one batched search
@@ -296,9 +301,35 @@ class LearnedSearch:
         fallback = torch.nonzero(~ok).flatten().to(torch.int32)
         return entry, steps.contiguous(), confident, fallback
 
+    def calibrate_fallback_cap(self, queries, pq, k=10, conf_threshold=0.9, quantile=0.999, max_queries=20000):
+        """Expansion cap for fallback queries: the `quantile` of the step at which the plain
+        search's top-k settles (its last change; the kernel's per-query stats), over training
+        queries the predictor sends to the fallback (all of them if fewer than 500 do).  A query
+        whose top-k settles within the cap returns exactly the plain search's top-k."""
+        import torch
+
+        qd = torch.from_numpy(np.ascontiguousarray(queries[:max_queries], dtype=np.float32)).to(self.dev)
+        _, conf, _ = self.predict(qd)
+        low = torch.nonzero(conf < conf_threshold).flatten()
+        if low.numel() >= 500:
+            qd = qd[low].contiguous()
+        n = qd.shape[0]
+        ids = torch.empty((n, k), dtype=torch.int32, device=self.dev)
+        flags = torch.empty(n, dtype=torch.int32, device=self.dev)
+        st = torch.empty((n, 8), dtype=torch.int32, device=self.dev)
+        self.index.search_device(qd, pq, k, ids, flags=flags, stats=st, mode=MODE_FP32)
+        torch.cuda.synchronize()
+        settle = st[:, 4].cpu().numpy().astype(np.int64)
+        cap = int(math.ceil(float(np.quantile(settle, quantile)))) if quantile < 1.0 else int(settle.max())
+        self.fallback_cap = max(cap, 1)
+        return {"cap": self.fallback_cap, "quantile": quantile, "queries": n,
+                "low_confidence_only": bool(low.numel() >= 500), "settle_mean": round(float(settle.mean()), 1),
+                "settle_max": int(settle.max()), "expansions_mean": round(float(st[:, 0].float().mean().item()), 1)}
+
     def search(self, q_dev, pq, k, ids, plan=None, pq_fallback=None, conf_threshold=0.9, step_scale=1.5, step_add=8,
-               storage=0, dists=None, stats=None):
-        """Learned search of a batch into ids (int32 cuda [nq, >=k]); returns the plan."""
+               storage=0, dists=None, stats=None, fallback_cap=None):
+        """Learned search of a batch into ids (int32 cuda [nq, >=k]); returns the plan.
+        fallback_cap: expansion cap of the fallback queries (None: uncapped plain search)."""
         if plan is None:
             plan = self.plan(q_dev, conf_threshold, step_scale, step_add)
         import torch
@@ -310,8 +341,9 @@ class LearnedSearch:
             side = self._side_stream()
             side.wait_stream(main)
             with torch.cuda.stream(side):
+                fsteps = None if fallback_cap is None else torch.full_like(steps, int(fallback_cap))
                 self.index.search_device(q_dev, pq_fallback or pq, k, ids, dists=dists, stats=stats,
-                                         subset=fallback, storage=storage, retry=0)
+                                         subset=fallback, storage=storage, retry=0, steps=fsteps)
         if confident.numel():
             self.index.search_device(q_dev, pq, k, ids, dists=dists, stats=stats, entry=entry, steps=steps,
                                      subset=confident, storage=storage, retry=0)
@@ -332,17 +364,18 @@ class LearnedBatch:
 
     Per query, on the GPU: confident -> (pq_learned, its neighbourhood's pivot vertices,
     expansion cap from the stop head); fallback -> (pq_fallback, vertex 0 -- entry list
-    padded with -1 --, no cap).  The batch has one tier per budget (tier knobs from the
+    padded with -1 --, fallback_cap or no cap).  The batch has one tier per budget (tier knobs from the
     autotuner or defaults).  run() enqueues the pipeline; capture() records it once in a
     CUDA/HIP graph and replay() relaunches it."""
 
     def __init__(self, ls, nq, pq_learned, pq_fallback, k=10, conf_threshold=0.9, step_scale=1.5, step_add=8,
-                 storage=0, mode=MODE_FP32, tiers=None, k_ld=None, **batch_kw):
+                 storage=0, mode=MODE_FP32, tiers=None, k_ld=None, fallback_cap=None, **batch_kw):
         import torch
 
         self.ls, self.nq, self.k = ls, nq, k
         self.pq_l, self.pq_f = int(pq_learned), int(pq_fallback)
         self.conf_threshold, self.step_scale, self.step_add = conf_threshold, step_scale, step_add
+        self.fallback_cap = fallback_cap
         dev = ls.dev
         lo, hi = sorted({self.pq_l, self.pq_f}) if self.pq_l != self.pq_f else (self.pq_l, None)
         self.tiers = tiers or ([tier(lo)] + ([tier(hi)] if hi else []))
@@ -359,7 +392,7 @@ class LearnedBatch:
 
         entry, conf, stop = self.ls.predict(q_dev)
         ok = conf >= self.conf_threshold
-        big = torch.full_like(stop, 1.0e6)
+        big = torch.full_like(stop, float(self.fallback_cap) if self.fallback_cap else 1.0e6)
         caps = torch.where(ok, (self.step_scale * (torch.exp2(stop) - 1.0) + self.step_add).clamp(1, 1.0e6), big)
         pq = torch.where(ok, torch.full_like(self.k_arr, self.pq_l), torch.full_like(self.k_arr, self.pq_f))
         entry = torch.where(ok[:, None], entry, torch.full_like(entry, -1))

@@ -195,3 +195,61 @@ def test_learned_batch_equals_split_search_and_graph_replays(mix50k):
         lb.replay()
         torch.cuda.synchronize()
         assert torch.equal(lb.ids, got), f"replay {rep}"
+
+
+def test_fallback_cap_keeps_settled_queries_equal_to_plain(mix50k):
+    """The fallback expansion cap calibrated on training queries: the cap is the quantile of the
+    plain search's settle steps over the low-confidence training queries; a fallback query is
+    flagged as capped when the uncapped plain search expands past the cap, and
+    every fallback query whose top-k settled within the cap returns the plain search's top-k.
+    The batched pipeline equals the split search under the same cap."""
+    import torch
+
+    base, graph, ix, train, test = mix50k
+    ls = learned.LearnedSearch(ix, base, n_pivots=256, hidden=64, seed=2)
+    ls.fit(train, pq_label=64, epochs=6, batch=512, verbose=False)
+    q = torch.from_numpy(test).cuda()
+    nq, k, pqf = q.shape[0], 10, 64
+    _, conf, _ = ls.predict(q)
+    thr = float(conf.quantile(0.7).item())  # a large fallback group
+    cal = ls.calibrate_fallback_cap(train, pqf, k=k, conf_threshold=thr, quantile=0.9, max_queries=5000)
+    print(f"[learned] fallback cap {cal}", flush=True)
+    # the calibration, restated: plain search of the same training queries, settle quantile
+    tq = torch.from_numpy(np.ascontiguousarray(train[:5000])).cuda()
+    _, tconf, _ = ls.predict(tq)
+    low = torch.nonzero(tconf < thr).flatten()
+    assert low.numel() >= 500 and cal["low_confidence_only"]
+    tq = tq[low].contiguous()
+    tst = torch.empty((tq.shape[0], 8), dtype=torch.int32, device="cuda")
+    tids = torch.empty((tq.shape[0], k), dtype=torch.int32, device="cuda")
+    ix.search_device(tq, pqf, k, tids, stats=tst, mode=song_amd.MODE_FP32)
+    torch.cuda.synchronize()
+    settle = tst[:, 4].cpu().numpy().astype(np.int64)
+    assert cal["cap"] == int(np.ceil(np.quantile(settle, 0.9)))
+    cap = cal["cap"]
+
+    lb = learned.LearnedBatch(ls, nq, pq_learned=24, pq_fallback=pqf, k=k, conf_threshold=thr, step_scale=1.5,
+                              mode=song_amd.MODE_FP32, fallback_cap=cap)
+    pq_arr, entry, caps, ok = lb.run(q)
+    torch.cuda.synchronize()
+    assert bool((caps[~ok] == cap).all())
+    fb_idx = torch.nonzero(~ok).flatten().to(torch.int32)
+    ref = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    st = torch.zeros((nq, 8), dtype=torch.int32, device="cuda")
+    ix.search_device(q, pqf, k, ref, stats=st, subset=fb_idx, mode=song_amd.MODE_FP32, retry=0)
+    torch.cuda.synchronize()
+    fb = fb_idx.long()
+    exp_plain, settle_plain = st[fb, 0], st[fb, 4]
+    capped = (lb.flags[fb] & song_amd.FLAG_STEP_CAP) != 0
+    print(f"[learned] {fb.numel()} fallback queries, {int(capped.sum())} capped at {cap}", flush=True)
+    assert int(capped.sum()) > 0
+    # flagged: the plain search expanded past the cap, or stopped right at it (the cap is tested
+    # before the stop rule)
+    assert bool((capped >= (exp_plain > cap)).all()) and bool((capped <= (exp_plain >= cap)).all())
+    settled = settle_plain <= cap
+    assert torch.equal(lb.ids[fb][settled], ref[fb][settled])
+    # the eager split search under the same cap gives the same ids
+    split = torch.full_like(lb.ids, -3)
+    ls.search(q, 24, k, split, conf_threshold=thr, step_scale=1.5, pq_fallback=pqf, fallback_cap=cap)
+    torch.cuda.synchronize()
+    assert torch.equal(split, lb.ids)
