@@ -269,13 +269,22 @@ def bench_c2(args, torch, song_amd):
             log(f"fallback cap {fcap} expansions (settle quantile {args.fallback_quantile} over {fcal['queries']} "
                 f"training queries, settle mean {fcal['settle_mean']}, max {fcal['settle_max']})")
         lst = torch.empty((nq, 8), dtype=torch.int32, device=dev)
+
+        def ltiers(lpq):
+            # the learned queries read BF16 rows (the form's storage); the fallback tier is the plain
+            # search and reads its FP32 rows
+            if lpq >= pq:
+                return None
+            return [song_amd.tier(lpq), song_amd.tier(pq, storage=song_amd.STORE_F32)]
+
         lsweep = []
         best = None
         for lpq in [int(x) for x in args.learned_pqs.split(",")]:
             for sc in [float(x) for x in args.step_scales.split(",")]:
                 scale = sc if sc > 0 else 1e6
                 lb = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                          storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap)
+                                          storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap,
+                                          tiers=ltiers(lpq))
                 plan = lb.run(q, stats=lst)
                 torch.cuda.synchronize()
                 rl = recall(lb.ids)
@@ -295,11 +304,12 @@ def bench_c2(args, torch, song_amd):
             ent, lpq, scale, stl = best
             tt = time.time()
             lb0 = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                       storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap)
+                                       storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap,
+                                       tiers=ltiers(lpq))
             p = lb0.run(q)
             table = autotune.tune(ix, q, p[0], lb0.k_arr, tiers=lb0.tiers, mode=song_amd.MODE_FP32,
                                   storage=song_amd.STORE_BF16, reps=3, max_queries_per_tier=nq, k_ld=k, entry=p[1],
-                                  steps=p[2])
+                                  steps=p[2], freeze=("storage",))
             torch.cuda.synchronize()
             eager_ids = lb0.ids.clone()
             lb0.batch.close()
@@ -344,8 +354,9 @@ def bench_c2(args, torch, song_amd):
     have_learned = bool(learned_info and "qps" in learned_info)
     if have_learned:
         value, ms_step, dtype = learned_info["qps"], learned_info["ms_per_step"], "bf16"
-        form = ("learned + BF16, batched (MLP start vertices and stop caps, BF16 rows, exact FP32 re-rank, "
-                "confidence fallback, persistent tiers, one graph replay per step)")
+        form = ("learned + BF16, batched (MLP start vertices and stop caps, BF16 rows, exact FP32 re-rank; "
+                "confidence fallback to the plain search on FP32 rows under a training-calibrated expansion cap; "
+                "persistent tiers, one graph replay per step)")
     else:
         value, ms_step, dtype = round(qps_plain, 1), ms_plain, "fp32"
         form = "plain (bit-exact ids)"

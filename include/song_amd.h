@@ -183,6 +183,8 @@ typedef struct {
     int groups_per_cu;   /* persistent grid = groups_per_cu x compute units; 0 = all that fit */
     int visited_bits;    /* visited-table slot width: 16 (compact), 32, or 0 = compact whenever the
                             ids fit; results do not depend on it */
+    int storage;         /* rows this tier reads: SONG_STORE_*, or -1 = the batch's cfg.storage.  A
+                            query re-run after an overflow is searched with the retry tier's storage */
 } song_tier;
 
 typedef struct {

@@ -41,7 +41,8 @@ class SearchOpts(ctypes.Structure):
 
 class Tier(ctypes.Structure):
     _fields_ = [("pq_max", ctypes.c_int), ("hash_log2", ctypes.c_int), ("slack", ctypes.c_int), ("shape", ctypes.c_int),
-                ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int), ("visited_bits", ctypes.c_int)]
+                ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int), ("visited_bits", ctypes.c_int),
+                ("storage", ctypes.c_int)]
 
 
 MAX_TIERS = 8
@@ -234,13 +235,14 @@ def resolve_mode(index, queries, stream=None):
     return m.value
 
 
-TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu", "visited_bits")
+TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu", "visited_bits", "storage")
 
 
-def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0, visited_bits=0):
-    """One tier of a batch configuration (see song_tier in include/song_amd.h)."""
+def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0, visited_bits=0, storage=-1):
+    """One tier of a batch configuration (see song_tier in include/song_amd.h); storage -1 = the
+    batch's row storage."""
     return dict(pq_max=pq_max, hash_log2=hash_log2, slack=slack, shape=shape, waves_per_group=waves_per_group,
-                groups_per_cu=groups_per_cu, visited_bits=visited_bits)
+                groups_per_cu=groups_per_cu, visited_bits=visited_bits, storage=storage)
 
 
 class Batch:
@@ -260,9 +262,11 @@ class Batch:
         cfg.n_tiers = len(tiers)
         cfg.parallel_tiers = 1 if parallel_tiers else 0
         cfg.priority_first = 1 if priority_first else 0
+        default = Tier()
+        ctypes.pointer(default)[0] = cfg.tier[0]  # the library's default tier, before tier 0 is filled
         for i, t in enumerate(tiers):
             for key in TIER_KEYS:
-                setattr(cfg.tier[i], key, int(t.get(key, getattr(cfg.tier[0], key) if key != "pq_max" else 0)))
+                setattr(cfg.tier[i], key, int(t.get(key, getattr(default, key) if key != "pq_max" else 0)))
         if retry is not None:
             for key in TIER_KEYS:
                 if key in retry:

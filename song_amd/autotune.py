@@ -11,6 +11,9 @@ coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
   groups_per_cu    persistent grid size (0 = all that fit)
   visited_bits     visited-table slot width (0 = compact 16-bit slots where the ids fit, 32 = 32-bit:
                    fewer instructions per probe, twice the LDS per slot)
+  storage          with low-precision batch storage: the tier's rows (-1 = the batch's, or FP32 rows,
+                   whose decode-free distances shorten a latency-bound tier); taken only where the
+                   results stay identical, e.g. BF16 rows of integer data
 
 Each candidate runs the tier's queries through song_batch_run and is timed with events
 (median of `reps`); a candidate is accepted only if every result equals the default
@@ -54,8 +57,9 @@ def _neighbours(value, choices):
 
 
 def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STORE_F32, rerank=1, reps=3,
-         max_queries_per_tier=20000, k_ld=None, entry=None, steps=None, verbose=False):
-    """Tune every tier on its own queries; returns the table (dict)."""
+         max_queries_per_tier=20000, k_ld=None, entry=None, steps=None, verbose=False, freeze=()):
+    """Tune every tier on its own queries; returns the table (dict).  freeze: knob names kept at
+    the given tiers' values (e.g. "storage", where the rows are part of what is measured)."""
     import torch
 
     t_start = time.time()
@@ -103,13 +107,18 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         # two rounds of coordinate descent: a knob's best value can depend on another's (a larger
         # visited table only pays off with fewer waves per group, which the first round may have
         # rejected at the smaller table)
+        knobs = [("waves_per_group", (1, 2, 4)),
+                 ("hash_log2", tuple(range(8, 15))),
+                 ("shape", SHAPES),
+                 ("slack", (0, 8, 32, 128)),
+                 ("groups_per_cu", (0, 1, 2, 4, 8)),
+                 ("visited_bits", (0, 32))]
+        if storage != STORE_F32:
+            knobs.append(("storage", (-1, STORE_F32)))
+        knobs = [kc for kc in knobs if kc[0] not in freeze]
+        best.setdefault("storage", -1)
         for _round in range(2):
-            for key, choices in (("waves_per_group", (1, 2, 4)),
-                                 ("hash_log2", tuple(range(8, 15))),
-                                 ("shape", SHAPES),
-                                 ("slack", (0, 8, 32, 128)),
-                                 ("groups_per_cu", (0, 1, 2, 4, 8)),
-                                 ("visited_bits", (0, 32))):
+            for key, choices in knobs:
                 cur = best[key]
                 if key == "hash_log2" and cur == 0:
                     cur = -1

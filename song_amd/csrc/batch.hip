@@ -128,6 +128,7 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     t.waves_per_group = 1;
     t.groups_per_cu = 0;
     t.visited_bits = 0;
+    t.storage = -1;
     c->tier[0] = t;
     c->retry = t;
     c->retry.hash_log2 = 0;  // the largest tier's filter x 4
@@ -162,6 +163,8 @@ static int check_tier(const song_tier &t, int idx) {
         return fail(SONG_E_ARG, "tier %d: groups_per_cu=%d", idx, t.groups_per_cu);
     if (t.visited_bits != 0 && t.visited_bits != 16 && t.visited_bits != 32)
         return fail(SONG_E_ARG, "tier %d: visited_bits=%d not 0, 16 or 32", idx, t.visited_bits);
+    if (t.storage < -1 || t.storage > SONG_STORE_FP8)
+        return fail(SONG_E_ARG, "tier %d: storage=%d not -1 or a SONG_STORE_* value", idx, t.storage);
     return SONG_OK;
 }
 
@@ -224,12 +227,13 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         if (!is_retry) top_hlog = std::max(top_hlog, hlog);
         Launch L;
         L.pq_max = ti.pq_max;
-        int rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
+        const int store = ti.storage >= 0 ? ti.storage : c.storage;
+        int rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
         // the re-run's default table (the largest tier's, 4x) may not fit one wave's LDS: take the
         // largest that does (the re-run also has a 4x larger global spill table)
         while (rc == SONG_E_OVERFLOW && is_retry && !ti.hash_log2 && hlog > top_hlog) {
             --hlog;
-            rc = make_plan(ix, exact, c.storage, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
+            rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
         }
         if (rc != SONG_OK) return cleanup(rc);
         L.fn = search_fn(ix->metric, L.plan, true);
@@ -301,8 +305,9 @@ static int check_args(const song_batch *b, const RunArgs &a) {
     if (a.qp.subset) return fail(SONG_E_ARG, "query subsets are not supported by batches");
     if (a.qp.entry && (a.qp.n_entry < 1 || a.qp.n_entry > 64))
         return fail(SONG_E_ARG, "n_entry=%d not in [1,64]", a.qp.n_entry);
-    if (b->cfg.storage != SONG_STORE_F32 && !ix->d_lp[b->cfg.storage])
-        return fail(SONG_E_ARG, "storage %d not built", b->cfg.storage);
+    for (const Launch &L : b->tiers)
+        if (L.plan.store != SONG_STORE_F32 && !ix->d_lp[L.plan.store])
+            return fail(SONG_E_ARG, "storage %d not built", L.plan.store);
     return SONG_OK;
 }
 

@@ -243,6 +243,30 @@ def test_batch_entries_steps_and_bf16(c1):
         assert (got[2] & song_amd.FLAG_STEP_CAP).sum() > 0
 
 
+def test_batch_per_tier_storage(c1):
+    """A tier's own row storage: with BF16 batch storage, the middle tier reads FP32 rows; every
+    query equals the plain search at its tier's storage (non-integer cosine data, so the two
+    storages give different results and a mix-up shows)."""
+    import torch
+
+    ix = c1["ix"]
+    ix.add_storage(song_amd.STORE_BF16)
+    nq = 1500
+    q = c1["q"][:nq].contiguous()
+    k, pq = c1["k"][:nq], c1["pq"][:nq]
+    w_bf = _plain(ix, q, k, pq, song_amd.MODE_FP32, storage=song_amd.STORE_BF16)
+    w_32 = _plain(ix, q, k, pq, song_amd.MODE_FP32, storage=song_amd.STORE_F32)
+    mid = torch.from_numpy((pq > 64) & (pq <= 256)).to(q.device)
+    assert bool(mid.any()) and bool((~mid).any())
+    # the storages differ on these queries
+    assert not (torch.equal(w_bf[0][mid], w_32[0][mid]) and torch.equal(w_bf[1][mid], w_32[1][mid]))
+    want = tuple(torch.where(mid[:, None] if a.dim() == 2 else mid, a, b_) for a, b_ in zip(w_32, w_bf))
+    b = song_amd.Batch(ix, [tier(64), tier(256, waves_per_group=2, storage=song_amd.STORE_F32), tier(512)], nq,
+                       mode=song_amd.MODE_FP32, storage=song_amd.STORE_BF16)
+    _same(_batch_run(b, q, c1["k_d"][:nq], c1["pq_d"][:nq]), want, "per-tier storage")
+    _same(_batch_run(b, q, c1["k_d"][:nq], c1["pq_d"][:nq], graph=True), want, "per-tier storage, graph")
+
+
 def test_autotuned_tiers_keep_results(c1):
     import torch
 
