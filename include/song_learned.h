@@ -70,6 +70,19 @@ int song_mlp_infer_nbr(song_mlp *mlp, const uint16_t *d_x, int batch, const int 
 int song_mlp_features(const float *d_queries, int nq, int dim, const float *d_mean, const float *d_inv_std,
                       const float *d_v0, const float *fscale_host, uint16_t *d_x, int width, void *stream);
 
+/*
+ * Search plan of a learned batch from song_mlp_infer_nbr's outputs, one thread per query.
+ * Query i is confident when d_conf[i] >= conf_threshold and its best pivot (d_top2[2i]) is one of
+ * the n_pivots: d_pq[i] = pq_learned, its d_entry row
+ * (n_nbr ints) = d_pivot_vertex[d_nbr[top1 * n_nbr + e]], d_caps[i] = (int) of the float32
+ * clamp(step_scale * (2^d_stop[i] - 1) + step_add, 1, 1e6).  Otherwise d_pq[i] = pq_fallback,
+ * the entry row is -1 (vertex 0) and d_caps[i] = fallback_cap (1e6 when 0).  d_ok[i] = 1 or 0.
+ */
+int song_mlp_plan(const int *d_top2, const float *d_conf, const float *d_stop, int batch, const int *d_nbr, int n_nbr,
+                  const int *d_pivot_vertex, int n_pivots, float conf_threshold, float step_scale, float step_add,
+                  int fallback_cap, int pq_learned, int pq_fallback, int *d_pq, int *d_entry, int *d_caps,
+                  unsigned char *d_ok, void *stream);
+
 /* Generic C = alpha * A . B (+ bias) on bf16 inputs, float32 output (test hook for the GEMM). */
 int song_gemm_bf16(int M, int N, int K, const uint16_t *d_a, int sam, int sak, const uint16_t *d_b, int sbk, int sbn,
                    float *d_c, int ldc, float alpha, void *stream);

@@ -263,7 +263,11 @@ __global__ __launch_bounds__(64) void features_kernel(const float *Q, int dim, c
 
 // ---------------------------------------------------------------- fused inference
 
-constexpr int IQ = 64;  // queries per workgroup
+// queries per workgroup: 32 rows (MI = 2 row tiles of 16) keep the 4 waves' LDS to ~45 KB, so
+// a 10k-query batch launches 313 workgroups, two or three per CU (64 rows held the batch to 157
+// CUs at one wave per SIMD, and the output layer waited on its weight loads)
+constexpr int IQ = 32;
+constexpr int MI = IQ / 16;
 
 struct InferArgs {
     const uint16_t *X;
@@ -276,16 +280,17 @@ struct InferArgs {
     int n_nbr;
 };
 
-// One dense layer on the LDS tile: accumulators of in[64 rows] . W[n]^T for the 64-column
-// slice [n0, n0+64) of this wave (4x4 MFMA tiles).  K is a multiple of 16, so every
+// One dense layer on the LDS tile: accumulators of in[IQ rows] . W[n]^T for the 64-column
+// slice [n0, n0+64) of this wave (MI x 4 MFMA tiles).  K is a multiple of 16, so every
 // fragment (4 consecutive k of one row or column) is one aligned 8-byte load; the next
-// k-step's weight fragments are loaded while the current step's MFMAs run.
+// k-step's weight fragments are loaded while the current step's MFMAs run (a deeper ring of
+// in-flight weight loads measured no faster).
 __device__ __forceinline__ void tile_layer(const uint16_t *in, int ldin, int K, const uint16_t *W, int N, int n0,
-                                           v4f (&acc)[4][4]) {
+                                           v4f (&acc)[MI][4]) {
     const int lane = threadIdx.x & 63;
     const int kq = 4 * (lane >> 4);
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
     const uint16_t *wrow[4];
@@ -306,7 +311,7 @@ __device__ __forceinline__ void tile_layer(const uint16_t *in, int ldin, int K, 
         for (int ni = 0; ni < 4; ++ni)
             nb[ni] = (ok[ni] && more) ? *reinterpret_cast<const v4s *>(wrow[ni] + k0 + 16) : v4s{0, 0, 0, 0};
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi) {
+        for (int mi = 0; mi < MI; ++mi) {
             const v4s af = *reinterpret_cast<const v4s *>(arow + mi * 16 * ldin + k0);
 #pragma unroll
             for (int ni = 0; ni < 4; ++ni)
@@ -324,27 +329,49 @@ struct Soft {
 };
 __device__ __forceinline__ bool better(float v, int a, float w, int b) { return v > w || (v == w && a < b); }
 __device__ __forceinline__ Soft soft_empty() { return Soft{-INFINITY, -INFINITY, 0.0f, 0x7fffffff, 0x7fffffff}; }
-__device__ __forceinline__ void soft_add(Soft &t, float v, int a) {
-    if (better(v, a, t.v1, t.a1)) {
-        t.s = (t.v1 == -INFINITY ? 0.0f : t.s * expf(t.v1 - v)) + 1.0f;
-        t.v2 = t.v1;
-        t.a2 = t.a1;
-        t.v1 = v;
-        t.a1 = a;
-    } else {
-        t.s += expf(v - t.v1);
-        if (better(v, a, t.v2, t.a2)) {
-            t.v2 = v;
-            t.a2 = a;
-        }
-    }
+// the (value, index) pair into the top two, without branches
+__device__ __forceinline__ void top2_add(Soft &t, float v, int a) {
+    const bool b1 = better(v, a, t.v1, t.a1), b2 = better(v, a, t.v2, t.a2);
+    const float nv2 = b1 ? t.v1 : (b2 ? v : t.v2);
+    const int na2 = b1 ? t.a1 : (b2 ? a : t.a2);
+    t.v1 = b1 ? v : t.v1;
+    t.a1 = b1 ? a : t.a1;
+    t.v2 = nv2;
+    t.a2 = na2;
+}
+// four values of one row at once (-inf where absent): one rescale of the running sum to the new
+// maximum, four exponentials, then the top two
+__device__ __forceinline__ void soft_add4(Soft &t, const float (&v)[4], const int (&a)[4]) {
+    const float m = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
+    if (m == -INFINITY) return;
+    const float ref = fmaxf(m, t.v1);
+    float s = t.v1 == -INFINITY ? 0.0f : t.s * expf(t.v1 - ref);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s += expf(v[i] - ref);  // exp(-inf) = 0
+    t.s = s;
+    // the sum is kept relative to the top value: ref becomes v1 below whenever m > v1
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (v[i] != -INFINITY) top2_add(t, v[i], a[i]);
 }
 __device__ __forceinline__ Soft soft_merge(const Soft &x, const Soft &y) {
     if (y.v1 == -INFINITY) return x;
     if (x.v1 == -INFINITY) return y;
     Soft t;
     const bool xf = better(x.v1, x.a1, y.v1, y.a1);
-    const Soft &hi = xf ? x : y, &lo = xf ? y : x;
+    // field by field (a conditional over the two structs would select their addresses, which
+    // puts them in scratch memory)
+    Soft hi, lo;
+    hi.v1 = xf ? x.v1 : y.v1;
+    hi.v2 = xf ? x.v2 : y.v2;
+    hi.s = xf ? x.s : y.s;
+    hi.a1 = xf ? x.a1 : y.a1;
+    hi.a2 = xf ? x.a2 : y.a2;
+    lo.v1 = xf ? y.v1 : x.v1;
+    lo.v2 = xf ? y.v2 : x.v2;
+    lo.s = xf ? y.s : x.s;
+    lo.a1 = xf ? y.a1 : x.a1;
+    lo.a2 = xf ? y.a2 : x.a2;
     t.v1 = hi.v1;
     t.a1 = hi.a1;
     t.s = hi.s + lo.s * expf(lo.v1 - hi.v1);
@@ -377,7 +404,7 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
     for (int r = tid; r < IQ; r += 256) run[r] = soft_empty();
     __syncthreads();
 
-    v4f acc[4][4];
+    v4f acc[MI][4];
     // hidden layers: the 4 waves split the hidden units in slices of 64
     for (int layer = 0; layer < 2; ++layer) {
         const uint16_t *in = layer == 0 ? X : H1;
@@ -388,7 +415,7 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
         for (int n0 = wave * 64; n0 < a.hidden; n0 += 256) {
             tile_layer(in, ldin, K, W, a.hidden, n0, acc);
 #pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
+            for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
                 for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
@@ -404,32 +431,37 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
     // and the sum) for each of its 16 rows over the columns it owns; the 16 lanes of a row and
     // the 4 waves combine once at the end
     const int NO = a.C + 1;
-    Soft ts[4][4];
+    Soft ts[MI][4];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
         for (int j = 0; j < 4; ++j) ts[mi][j] = soft_empty();
     for (int c0 = 0; c0 < NO; c0 += 256) {
         const int n0 = c0 + wave * 64;
         tile_layer(H2, ldh, a.hidden, a.W3, NO, n0, acc);
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
+        for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int r = mi * 16 + 4 * (lane >> 4) + j;
+                float vv[4];
+                int nn[4];
 #pragma unroll
                 for (int ni = 0; ni < 4; ++ni) {
                     const int n = n0 + ni * 16 + (lane & 15);
+                    nn[ni] = n;
+                    vv[ni] = -INFINITY;
                     if (n >= NO) continue;
                     const float v = acc[mi][ni][j] + a.b3[n];
                     if (a.logits && r0 + r < a.batch) a.logits[(long long)(r0 + r) * NO + n] = v;
                     if (n == a.C) stp[r] = v;
-                    else soft_add(ts[mi][j], v, n);
+                    else vv[ni] = v;
                 }
+                soft_add4(ts[mi][j], vv, nn);
             }
     }
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int r = mi * 16 + 4 * (lane >> 4) + j;
@@ -454,14 +486,15 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
     }
     __syncthreads();
     // confidence: softmax mass of the best pivot's neighbourhood, its logits recomputed from
-    // the last hidden layer by 4 threads per row (entries e = quarter, quarter + 4, ...);
+    // the last hidden layer by TPR threads per row (entries e = quarter, quarter + TPR, ...);
     // without neighbourhoods, the mass of the top two
     {
-        const int row = tid >> 2, quarter = tid & 3;
+        constexpr int TPR = 256 / IQ;
+        const int row = tid / TPR, quarter = tid % TPR;
         const Soft t = run[row];
         float mass = 0.0f;
         if (a.nbr && t.a1 >= 0 && t.a1 < a.C) {  // (no valid best pivot: non-finite logits)
-            for (int e = quarter; e < a.n_nbr; e += 4) {
+            for (int e = quarter; e < a.n_nbr; e += TPR) {
                 const int p = a.nbr[(long long)t.a1 * a.n_nbr + e];
                 const uint16_t *w = a.W3 + (long long)p * a.hidden;
                 const uint16_t *hrow = H2 + row * ldh;
@@ -477,8 +510,8 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
         } else if (!a.nbr && quarter == 0) {
             mass = 1.0f + (t.v2 == -INFINITY ? 0.0f : expf(t.v2 - t.v1));
         }
-        mass += __shfl_xor(mass, 1);
-        mass += __shfl_xor(mass, 2);
+#pragma unroll
+        for (int o = 1; o < TPR; o <<= 1) mass += __shfl_xor(mass, o);
         if (quarter == 0 && r0 + row < a.batch) {
             a.top2[2 * (r0 + row)] = t.a1;
             a.top2[2 * (r0 + row) + 1] = t.a2;
@@ -720,6 +753,40 @@ extern "C" int song_mlp_infer_nbr(song_mlp *m, const uint16_t *d_x, int batch, c
     const size_t lds = (size_t)IQ * (m->width + 8) * 2 + 2 * (size_t)IQ * (m->hidden + 8) * 2 + (5 * IQ) * sizeof(Soft) + IQ * 4;
     if (lds > 65536) MHIP(hipFuncSetAttribute((const void *)infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(infer_kernel, dim3((batch + IQ - 1) / IQ), dim3(256), lds, (hipStream_t)stream, a);
+    MHIP(hipGetLastError());
+    return 0;
+}
+
+__global__ __launch_bounds__(256) void plan_kernel(const int *top2, const float *conf, const float *stop, int batch,
+                                                   const int *nbr, int n_nbr, const int *pivot_vertex, int npiv, float thr,
+                                                   float scale, float add, int fcap, int pq_l, int pq_f, int *pq,
+                                                   int *entry, int *caps, unsigned char *ok) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= batch) return;
+    const int best = top2[2 * i];
+    // no valid best pivot (non-finite logits): the fallback
+    const bool good = conf[i] >= thr && best >= 0 && best < npiv;
+    int *row = entry + (long long)i * n_nbr;
+    for (int e = 0; e < n_nbr; ++e) row[e] = good ? pivot_vertex[nbr[(long long)best * n_nbr + e]] : -1;
+    const float c = fminf(fmaxf(scale * (exp2f(stop[i]) - 1.0f) + add, 1.0f), 1.0e6f);
+    caps[i] = good ? (int)c : (fcap > 0 ? fcap : 1000000);
+    pq[i] = good ? pq_l : pq_f;
+    ok[i] = good ? 1 : 0;
+}
+
+extern "C" int song_mlp_plan(const int *d_top2, const float *d_conf, const float *d_stop, int batch, const int *d_nbr,
+                             int n_nbr, const int *d_pivot_vertex, int n_pivots, float conf_threshold,
+                             float step_scale, float step_add, int fallback_cap, int pq_learned, int pq_fallback, int *d_pq,
+                             int *d_entry, int *d_caps, unsigned char *d_ok, void *stream) {
+    if (!d_top2 || !d_conf || !d_stop || !d_nbr || !d_pivot_vertex || !d_pq || !d_entry || !d_caps || !d_ok)
+        return mfail(-1, "null argument");
+    if (n_nbr < 1 || pq_learned < 1 || pq_fallback < 1 || fallback_cap < 0)
+        return mfail(-1, "bad plan arguments n_nbr=%d pq=%d/%d cap=%d", n_nbr, pq_learned, pq_fallback, fallback_cap);
+    if (batch < 1) return 0;
+    hipLaunchKernelGGL(plan_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, (hipStream_t)stream, d_top2,
+                       d_conf, d_stop, batch, d_nbr, n_nbr, d_pivot_vertex, n_pivots, conf_threshold, step_scale,
+                       step_add,
+                       fallback_cap, pq_learned, pq_fallback, d_pq, d_entry, d_caps, d_ok);
     MHIP(hipGetLastError());
     return 0;
 }

@@ -57,6 +57,8 @@ def _mlp_lib():
         L.song_mlp_infer.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
         L.song_mlp_infer_nbr.argtypes = [vp, vp, i32, vp, i32, vp, vp, vp, vp, vp]
         L.song_mlp_features.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, i32, vp]
+        f32 = ctypes.c_float
+        L.song_mlp_plan.argtypes = [vp, vp, vp, i32, vp, i32, vp, i32, f32, f32, f32, i32, i32, i32, vp, vp, vp, vp, vp]
         L.song_gemm_bf16.argtypes = [i32, i32, i32, vp, i32, i32, vp, i32, i32, vp, i32, ctypes.c_float, vp]
         L._mlp_bound = True
     return L
@@ -290,16 +292,33 @@ class LearnedSearch:
         entry = self.pivot_vertex[self.nbr[top2[:, 0].long()].long()].contiguous()
         return entry, conf, stop
 
-    def plan(self, q_dev, conf_threshold=0.9, step_scale=1.5, step_add=8):
-        """Entries, expansion caps and the confident / fallback split for a batch."""
+    def plan_arrays(self, q_dev, conf_threshold, step_scale, step_add, fallback_cap, pq_learned, pq_fallback):
+        """(pq [nq], entries [nq, n_nbr] (-1 rows for the fallback), expansion caps [nq], confident
+        mask) on the GPU: features, fused inference, then one plan kernel (song_mlp_plan)."""
         import torch
 
-        entry, conf, stop = self.predict(q_dev)
-        steps = (step_scale * (torch.exp2(stop) - 1.0) + step_add).clamp(1, 1_000_000).to(torch.int32)
-        ok = conf >= conf_threshold
+        x = self.features(q_dev)
+        top2, conf, stop = self.mlp.infer(x, nbr=self.nbr)
+        nq = q_dev.shape[0]
+        pq = torch.empty(nq, dtype=torch.int32, device=self.dev)
+        entry = torch.empty((nq, self.nbr.shape[1]), dtype=torch.int32, device=self.dev)
+        caps = torch.empty(nq, dtype=torch.int32, device=self.dev)
+        ok = torch.empty(nq, dtype=torch.bool, device=self.dev)
+        _mcheck(_mlp_lib().song_mlp_plan(_dp(top2), _dp(conf), _dp(stop), nq, _dp(self.nbr), self.nbr.shape[1],
+                                         _dp(self.pivot_vertex), self.pivot_vertex.shape[0], float(conf_threshold),
+                                         float(step_scale), float(step_add), int(fallback_cap or 0), int(pq_learned),
+                                         int(pq_fallback), _dp(pq), _dp(entry), _dp(caps), _dp(ok), _stream()))
+        return pq, entry, caps, ok
+
+    def plan(self, q_dev, conf_threshold=0.9, step_scale=1.5, step_add=8):
+        """Entries, expansion caps and the confident / fallback split for a batch (entries and caps
+        are used for the confident queries)."""
+        import torch
+
+        _, entry, caps, ok = self.plan_arrays(q_dev, conf_threshold, step_scale, step_add, 0, 1, 1)
         confident = torch.nonzero(ok).flatten().to(torch.int32)
         fallback = torch.nonzero(~ok).flatten().to(torch.int32)
-        return entry, steps.contiguous(), confident, fallback
+        return entry, caps, confident, fallback
 
     def calibrate_fallback_cap(self, queries, pq, k=10, conf_threshold=0.9, quantile=0.999, max_queries=20000):
         """Expansion cap for fallback queries: the `quantile` of the step at which the plain
@@ -388,15 +407,8 @@ class LearnedBatch:
 
     def plan(self, q_dev):
         """(pq [nq], entries [nq, E] with -1 padding, caps [nq], confident mask) on the GPU."""
-        import torch
-
-        entry, conf, stop = self.ls.predict(q_dev)
-        ok = conf >= self.conf_threshold
-        big = torch.full_like(stop, float(self.fallback_cap) if self.fallback_cap else 1.0e6)
-        caps = torch.where(ok, (self.step_scale * (torch.exp2(stop) - 1.0) + self.step_add).clamp(1, 1.0e6), big)
-        pq = torch.where(ok, torch.full_like(self.k_arr, self.pq_l), torch.full_like(self.k_arr, self.pq_f))
-        entry = torch.where(ok[:, None], entry, torch.full_like(entry, -1))
-        return pq.contiguous(), entry.contiguous(), caps.to(torch.int32).contiguous(), ok
+        return self.ls.plan_arrays(q_dev, self.conf_threshold, self.step_scale, self.step_add, self.fallback_cap,
+                                   self.pq_l, self.pq_f)
 
     def run(self, q_dev, stats=None, dists=None):
         """Enqueue the whole pipeline for q_dev (nq queries); returns the plan."""

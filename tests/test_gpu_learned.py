@@ -253,3 +253,46 @@ def test_fallback_cap_keeps_settled_queries_equal_to_plain(mix50k):
     ls.search(q, 24, k, split, conf_threshold=thr, step_scale=1.5, pq_fallback=pqf, fallback_cap=cap)
     torch.cuda.synchronize()
     assert torch.equal(split, lb.ids)
+
+
+def test_plan_kernel_matches_its_restatement():
+    """song_mlp_plan against the tensor formula it replaces: confident = conf >= threshold with a
+    valid best pivot; entries from the best pivot's neighbourhood, caps from the stop output
+    (integer stop values, so 2^stop is exact in both), fallback rows -1 / the fallback cap."""
+    import ctypes
+    import torch
+
+    dev = torch.device("cuda")
+    g = torch.Generator(device="cuda").manual_seed(3)
+    B, C, E = 3000, 500, 8
+    top2 = torch.randint(0, C, (B, 2), device=dev, dtype=torch.int32, generator=g)
+    top2[::97, 0] = 0x7FFFFFFF  # no valid best pivot (non-finite logits)
+    top2[::89, 0] = -1
+    conf = torch.rand(B, device=dev, generator=g)
+    conf[::13] = 0.9  # exactly at the threshold: confident
+    stop = torch.randint(0, 12, (B,), device=dev, generator=g).float()
+    nbr = torch.randint(0, C, (C, E), device=dev, dtype=torch.int32, generator=g)
+    pv = torch.randint(0, 10 ** 6, (C,), device=dev, dtype=torch.int32, generator=g)
+    L = learned._mlp_lib()
+    for cap, scale in ((0, 1.5), (700, 2.5)):
+        pq = torch.empty(B, dtype=torch.int32, device=dev)
+        entry = torch.empty((B, E), dtype=torch.int32, device=dev)
+        caps = torch.empty(B, dtype=torch.int32, device=dev)
+        ok = torch.empty(B, dtype=torch.bool, device=dev)
+        dp = learned._dp
+        learned._mcheck(L.song_mlp_plan(dp(top2), dp(conf), dp(stop), B, dp(nbr), E, dp(pv), C, ctypes.c_float(0.9),
+                                        ctypes.c_float(scale), ctypes.c_float(8.0), cap, 192, 768, dp(pq), dp(entry),
+                                        dp(caps), dp(ok), None))
+        torch.cuda.synchronize()
+        best = top2[:, 0].long()
+        valid = (best >= 0) & (best < C)
+        want_ok = (conf >= 0.9) & valid
+        safe = torch.where(valid, best, torch.zeros_like(best))
+        want_entry = torch.where(want_ok[:, None], pv[nbr[safe].long()], torch.full_like(entry, -1))
+        want_caps = torch.where(want_ok, (scale * (torch.exp2(stop) - 1.0) + 8.0).clamp(1, 1e6).to(torch.int32),
+                                torch.full_like(caps, cap if cap else 1000000))
+        want_pq = torch.where(want_ok, torch.full_like(pq, 192), torch.full_like(pq, 768))
+        assert torch.equal(ok, want_ok) and bool(want_ok.any()) and bool((~want_ok).any())
+        assert torch.equal(entry, want_entry)
+        assert torch.equal(caps, want_caps)
+        assert torch.equal(pq, want_pq)
