@@ -601,8 +601,9 @@ def main():
     ap.add_argument("--c3-pq", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries shard across GPUs "
-                                   "with no communication, see DESIGN.md)"}))
+        print(json.dumps({"error": "the search runs on one GPU; --gpus must be 1 (more GPUs would each search "
+                                   "their own share of the queries against a replica of the index, with no "
+                                   "communication)"}))
         sys.exit(2)
     if args.no_recall and not args.pq:
         ap.error("--no-recall needs --pq")
