@@ -273,6 +273,8 @@ struct InferArgs {
     const uint16_t *X;
     int batch, width, hidden, C;
     const uint16_t *W1, *W2, *W3;
+    const uint16_t *W3i;  // W3 in the k-quad interleaved layout, npad columns
+    int npad;
     const float *b1, *b2, *b3;
     int *top2;  // batch x 2 pivot indices (best first)
     float *conf, *stop, *logits;
@@ -310,6 +312,48 @@ __device__ __forceinline__ void tile_layer(const uint16_t *in, int ldin, int K, 
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni)
             nb[ni] = (ok[ni] && more) ? *reinterpret_cast<const v4s *>(wrow[ni] + k0 + 16) : v4s{0, 0, 0, 0};
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi) {
+            const v4s af = *reinterpret_cast<const v4s *>(arow + mi * 16 * ldin + k0);
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni)
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[ni], acc[mi][ni], 0, 0, 0);
+        }
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) bf[ni] = nb[ni];
+    }
+}
+
+// As tile_layer, with the weights in the k-quad interleaved layout (interleave_kernel): a lane
+// group's 16 fragments of one step are one contiguous 128-byte run (four full cache lines per
+// load instead of 32 bytes of sixteen).  The lanes take the same k as in tile_layer, so the
+// accumulators are identical.
+__device__ __forceinline__ void tile_layer_il(const uint16_t *in, int ldin, int K, const uint16_t *Wi, int N, int npad,
+                                              int n0, v4f (&acc)[MI][4]) {
+    const int lane = threadIdx.x & 63;
+    const int q = lane >> 4, kq = 4 * q;
+#pragma unroll
+    for (int a = 0; a < MI; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    const long long sstride = 16LL * npad;  // elements per k-step: 4 quads x npad columns x 4
+    const uint16_t *wb[4];
+    bool ok[4];
+    v4s bf[4];
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+        const int n = n0 + ni * 16 + (lane & 15);
+        ok[ni] = n < N;
+        wb[ni] = Wi + ((long long)q * npad + (ok[ni] ? n : 0)) * 4;
+        bf[ni] = ok[ni] ? *reinterpret_cast<const v4s *>(wb[ni]) : v4s{0, 0, 0, 0};
+    }
+    const uint16_t *arow = in + (lane & 15) * ldin + kq;
+    for (int k0 = 0, st = 0; k0 < K; k0 += 16, ++st) {
+        const bool more = k0 + 16 < K;
+        v4s nb[4];
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+            nb[ni] = (ok[ni] && more) ? *reinterpret_cast<const v4s *>(wb[ni] + (st + 1) * sstride) : v4s{0, 0, 0, 0};
 #pragma unroll
         for (int mi = 0; mi < MI; ++mi) {
             const v4s af = *reinterpret_cast<const v4s *>(arow + mi * 16 * ldin + k0);
@@ -438,7 +482,7 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
         for (int j = 0; j < 4; ++j) ts[mi][j] = soft_empty();
     for (int c0 = 0; c0 < NO; c0 += 256) {
         const int n0 = c0 + wave * 64;
-        tile_layer(H2, ldh, a.hidden, a.W3, NO, n0, acc);
+        tile_layer_il(H2, ldh, a.hidden, a.W3i, NO, a.npad, n0, acc);
 #pragma unroll
         for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -528,6 +572,8 @@ struct song_mlp {
     long long off[6] = {0}, cnt[6] = {0}, total = 0;  // W1 b1 W2 b2 W3 b3
     float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
     uint16_t *Wb[3] = {nullptr, nullptr, nullptr};
+    uint16_t *W3i = nullptr;  // inference copy of the output weights, k-quad interleaved (infer_kernel)
+    int npad = 0;             // its column count: C + 1 rounded up to 16
     uint16_t *H1 = nullptr, *H2 = nullptr, *dZ = nullptr, *dH2 = nullptr, *dH1 = nullptr;
     float *Z = nullptr, *loss = nullptr;
     int t = 0;
@@ -540,12 +586,29 @@ static uint64_t splitmix(uint64_t &s) {
     return z ^ (z >> 31);
 }
 
+// The output weights W [N][K] (bf16, row-major) in the layout the inference GEMM reads: fragment
+// (k-step s, k-quad q, column n) = W[n][16s + 4q .. +4) at Wi[((4s + q) * npad + n) * 4], zero for
+// n >= N, so the 16 columns one lane group reads per step are one aligned 128-byte run.
+__global__ __launch_bounds__(256) void interleave_kernel(const uint16_t *W, uint16_t *Wi, int N, int K, int npad) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)K * npad) return;
+    const int t = (int)(e & 3);
+    const long long g = e >> 2;
+    const int n = (int)(g % npad);
+    const int sq = (int)(g / npad);
+    const int k = 16 * (sq >> 2) + 4 * (sq & 3) + t;
+    Wi[e] = n < N ? W[(long long)n * K + k] : (uint16_t)0;
+}
+
 static int refresh_bf16(song_mlp *m, hipStream_t st) {
     for (int l = 0; l < 3; ++l) {
         const long long n = m->cnt[2 * l];
         hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m->P + m->off[2 * l],
                            m->Wb[l], n);
     }
+    const long long ni = (long long)m->hidden * m->npad;
+    hipLaunchKernelGGL(interleave_kernel, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, st, m->Wb[2], m->W3i,
+                       m->C + 1, m->hidden, m->npad);
     MHIP(hipGetLastError());
     return 0;
 }
@@ -589,6 +652,8 @@ extern "C" int song_mlp_create(int in_dim, int hidden, int n_classes, int max_ba
         return bad("params");
     for (int l = 0; l < 3; ++l)
         if (hipMalloc(&m->Wb[l], m->cnt[2 * l] * 2)) return bad("bf16 weights");
+    m->npad = (n_classes + 1 + 15) & ~15;
+    if (hipMalloc(&m->W3i, (size_t)m->npad * hidden * 2)) return bad("interleaved output weights");
     const long long B = max_batch, NO = n_classes + 1;
     if (hipMalloc(&m->H1, B * hidden * 2) || hipMalloc(&m->H2, B * hidden * 2) || hipMalloc(&m->dZ, B * NO * 2) ||
         hipMalloc(&m->dH2, B * hidden * 2) || hipMalloc(&m->dH1, B * hidden * 2) || hipMalloc(&m->Z, B * NO * 4) ||
@@ -609,8 +674,8 @@ extern "C" int song_mlp_create(int in_dim, int hidden, int n_classes, int max_ba
 
 extern "C" int song_mlp_destroy(song_mlp *m) {
     if (!m) return 0;
-    void *ptrs[] = {m->P, m->G, m->Mo, m->Vo, m->Wb[0], m->Wb[1], m->Wb[2], m->H1, m->H2, m->dZ, m->dH2, m->dH1,
-                    m->Z, m->loss};
+    void *ptrs[] = {m->P, m->G, m->Mo, m->Vo, m->Wb[0], m->Wb[1], m->Wb[2], m->W3i, m->H1, m->H2, m->dZ, m->dH2,
+                    m->dH1, m->Z, m->loss};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete m;
@@ -741,6 +806,8 @@ extern "C" int song_mlp_infer_nbr(song_mlp *m, const uint16_t *d_x, int batch, c
     a.W1 = m->Wb[0];
     a.W2 = m->Wb[1];
     a.W3 = m->Wb[2];
+    a.W3i = m->W3i;
+    a.npad = m->npad;
     a.b1 = m->P + m->off[1];
     a.b2 = m->P + m->off[3];
     a.b3 = m->P + m->off[5];
