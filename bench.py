@@ -278,7 +278,7 @@ def bench_c2(args, torch, song_amd):
             return [song_amd.tier(lpq), song_amd.tier(pq, storage=song_amd.STORE_F32)]
 
         lsweep = []
-        best = None
+        cands = []  # settings that reach the target recall
         for lpq in [int(x) for x in args.learned_pqs.split(",")]:
             for sc in [float(x) for x in args.step_scales.split(",")]:
                 scale = sc if sc > 0 else 1e6
@@ -297,11 +297,11 @@ def bench_c2(args, torch, song_amd):
                 lsweep.append(ent)
                 log(f"learned+bf16 pq={lpq} scale={sc} recall={rl:.4f} {ms:.2f} ms ({ent['qps']:.0f} q/s) "
                     f"fallback={fb:.3f} exp={ent['mean_expansions']}")
-                if rl >= args.target_recall and (best is None or ent["qps"] > best[0]["qps"]):
-                    best = (ent, lpq, scale, stl)
+                if rl >= args.target_recall:
+                    cands.append((ent, lpq, scale, stl))
                 lb.batch.close()
-        if best is not None:
-            ent, lpq, scale, stl = best
+        def final_measure(ent, lpq, scale, stl):
+            # autotuned tiers, the pipeline captured as one graph, timed replays
             tt = time.time()
             lb0 = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
                                        storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap,
@@ -334,12 +334,23 @@ def bench_c2(args, torch, song_amd):
             alg_l = alg_bytes(stl, dim, dim * 2 + 4, nq)
             fit_out = {kk: vv for kk, vv in fit.items() if kk != "loss"}
             fit_out["final_loss"] = fit["loss"][-1]
-            learned_info = dict(ent, qps=round(nq / (ms_l / 1e3), 1), ms_per_step=round(ms_l, 4),
-                                wall_ms_per_step=round(wall_l, 4), recall_graph=round(r_final, 4),
-                                graph_ids_equal_eager=same, algorithmic_bytes_per_step=alg_l, tiers=table["tiers"],
-                                fallback=fallback_info, training=fit_out)
+            info = dict(ent, qps=round(nq / (ms_l / 1e3), 1), ms_per_step=round(ms_l, 4),
+                        wall_ms_per_step=round(wall_l, 4), recall_graph=round(r_final, 4),
+                        graph_ids_equal_eager=same, algorithmic_bytes_per_step=alg_l, tiers=table["tiers"],
+                        fallback=fallback_info, training=fit_out)
+            log(f"learned+bf16 batched pq={lpq} scale={ent['step_scale']}: {info['qps']:.0f} q/s "
+                f"({info['qps'] / qps_plain:.2f}x plain), recall {r_final:.4f}, graph == eager: {same}")
+            lb.batch.close()
+            return info
+
+        # the sweep's eager timings are close; the fastest few go through the full measurement
+        cands.sort(key=lambda c: -c[0]["qps"])
+        finals = [final_measure(*c) for c in cands[:args.learned_finalists]]
+        if finals:
+            learned_info = max(finals, key=lambda f: f["qps"])
+            learned_info["finalists"] = [{kk: f[kk] for kk in ("pq", "step_scale", "qps", "recall@10")} for f in finals]
             log(f"learned+bf16 batched form: {learned_info['qps']:.0f} q/s ({learned_info['qps'] / qps_plain:.2f}x "
-                f"plain), recall {r_final:.4f}, graph == eager: {same}")
+                f"plain), recall {learned_info['recall_graph']:.4f}")
         learned_info = learned_info or {"error": "no learned setting reached the target recall"}
         learned_info["sweep"] = lsweep
 
@@ -585,6 +596,8 @@ def main():
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--learned-pqs", default="64,96,128,192,256")
     ap.add_argument("--step-scales", default="1.5,2.5,0")
+    ap.add_argument("--learned-finalists", type=int, default=3,
+                    help="learned settings (fastest in the sweep at the target recall) measured in full")
     ap.add_argument("--fallback-quantile", type=float, default=0.999,
                     help="fallback expansion cap = this quantile of the plain search's top-k settle step over "
                          "low-confidence training queries (<= 0: uncapped fallback)")
