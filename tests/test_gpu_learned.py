@@ -60,8 +60,14 @@ def test_mlp_gradients_match_oracle():
         assert err < 2e-2, (t, err)
 
 
-def test_fused_inference_matches_oracle():
-    m, x, xr, _, _, params, _, _ = _small_problem(seed=1)
+# (batch, classes, hidden): 70 rows leave the last 32-row group partial; 300 classes make two
+# 256-column chunks and a padded interleaved weight copy; 64 hidden units are four k-steps
+SHAPES = [(64, 50, 32), (70, 300, 64)]
+
+
+@pytest.mark.parametrize("B,classes,hidden", SHAPES)
+def test_fused_inference_matches_oracle(B, classes, hidden):
+    m, x, xr, _, _, params, _, _ = _small_problem(seed=1, hidden=hidden, classes=classes, B=B)
     top2, conf, stop, logits = m.infer(x, logits=True)
     _, _, Z = mlp_ref.forward(params, xr, emulate_bf16=True)
     order, rconf, rstop = mlp_ref.top2_conf(Z)
@@ -73,10 +79,11 @@ def test_fused_inference_matches_oracle():
     assert np.allclose(stop.cpu().numpy(), rstop, rtol=1e-3, atol=1e-3)
 
 
-def test_fused_inference_neighbourhood_confidence():
+@pytest.mark.parametrize("B,classes,hidden", SHAPES)
+def test_fused_inference_neighbourhood_confidence(B, classes, hidden):
     import torch
 
-    m, x, xr, _, _, params, _, _ = _small_problem(seed=2)
+    m, x, xr, _, _, params, _, _ = _small_problem(seed=2, hidden=hidden, classes=classes, B=B)
     C, E = m.n_classes, 3
     nbr = np.stack([(np.arange(C) + j * 7) % C for j in range(E)], 1).astype(np.int32)
     top2, conf, _ = m.infer(x, nbr=torch.from_numpy(nbr).cuda())
