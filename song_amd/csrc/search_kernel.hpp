@@ -594,12 +594,12 @@ template <class Key>
 __device__ __forceinline__ Key select64(const Key *arr, int n) {
     const int lane = lane_id();
     Key best = (lane < n) ? arr[lane] : Key::inf();
-    // the next block's load is in flight while this one is sorted or scanned
-    Key nxt = (64 + lane < n) ? arr[64 + lane] : Key::inf();
+    // two blocks' loads are in flight while one is sorted or scanned (two register slots, the
+    // loop unrolled by two, so no slot is copied while its load is in flight)
+    Key na = (64 + lane < n) ? arr[64 + lane] : Key::inf();
+    Key nb = (128 + lane < n) ? arr[128 + lane] : Key::inf();
     for (int k = 2; k <= 64; k <<= 1) best = bitonic_steps(best, k, k);
-    for (int b0 = 64; b0 < n; b0 += 64) {
-        Key v = nxt;
-        if (b0 + 64 < n) nxt = (b0 + 64 + lane < n) ? arr[b0 + 64 + lane] : Key::inf();
+    const auto take = [&](Key v) {
         uint64_t m = ballot(lt(v, best.rl(63)));
         if (__popcll(m) > SEL_INSERT) {
             for (int k = 2; k <= 64; k <<= 1) v = bitonic_steps(v, k, k);
@@ -612,6 +612,16 @@ __device__ __forceinline__ Key select64(const Key *arr, int n) {
                 const Key dn = shr1(best);
                 best = ksel(lane < r, best, ksel(lane == r, x, dn));
             }
+        }
+    };
+    for (int b0 = 64; b0 < n; b0 += 128) {
+        const Key va = na;
+        if (b0 + 128 < n) na = (b0 + 128 + lane < n) ? arr[b0 + 128 + lane] : Key::inf();
+        take(va);
+        if (b0 + 64 < n) {
+            const Key vb = nb;
+            if (b0 + 192 < n) nb = (b0 + 192 + lane < n) ? arr[b0 + 192 + lane] : Key::inf();
+            take(vb);
         }
     }
     return best;
@@ -669,21 +679,30 @@ __device__ uint64_t select_dk(const Key *arr, int n, int r, uint32_t *hist) {
 }
 
 // In-place compactions of arr[0..n) (order not kept); return the new count.  Entries only
-// move left, below the start of the block being read, and each block's load (issued one
-// block ahead) completes before any of that block's slots is written.
+// move left: the writes of block j land below the start of block j + 1, so they never touch
+// the two blocks whose loads are in flight (issued two blocks ahead into two register slots,
+// the loop unrolled by two), and block j's own slots are written only after its load is used.
 template <class Key, class Keep>
 __device__ __forceinline__ int compact(Key *arr, int n, Keep keep_fn) {
     const int lane = lane_id();
     int out = 0;
-    Key nxt = (lane < n) ? arr[lane] : Key::inf();
-    for (int b0 = 0; b0 < n; b0 += 64) {
-        const bool v = b0 + lane < n;
-        const Key e = nxt;
-        if (b0 + 64 < n) nxt = (b0 + 64 + lane < n) ? arr[b0 + 64 + lane] : Key::inf();
-        const bool keep = v && keep_fn(e);
+    Key na = (lane < n) ? arr[lane] : Key::inf();
+    Key nb = (64 + lane < n) ? arr[64 + lane] : Key::inf();
+    const auto put = [&](int b0, const Key &e) {
+        const bool keep = b0 + lane < n && keep_fn(e);
         const uint64_t km = ballot(keep);
         if (keep) arr[out + lanes_below(km)] = e;
         out += __popcll(km);
+    };
+    for (int b0 = 0; b0 < n; b0 += 128) {
+        const Key ea = na;
+        if (b0 + 128 < n) na = (b0 + 128 + lane < n) ? arr[b0 + 128 + lane] : Key::inf();
+        put(b0, ea);
+        if (b0 + 64 < n) {
+            const Key eb = nb;
+            if (b0 + 192 < n) nb = (b0 + 192 + lane < n) ? arr[b0 + 192 + lane] : Key::inf();
+            put(b0 + 64, eb);
+        }
     }
     return out;
 }
