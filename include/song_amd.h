@@ -71,7 +71,7 @@ enum { SONG_STORE_F32 = 0, SONG_STORE_BF16 = 1, SONG_STORE_FP8 = 2 };
 
 typedef struct {
     int mode;      /* SONG_MODE_* (default AUTO); BF16/FP8 storage always accumulates in FP32 */
-    int slack;     /* queue entries kept beyond pq for distance ties (default 32) */
+    int slack;     /* queue entries kept beyond pq for distance ties (default 128: fewer bag truncations) */
     int hash_log2; /* visited-table slots = 2^hash_log2; 0 = sized from pq */
     int max_steps; /* expansion cap; 0 = 1,000,000 (graph.h:183) */
     int retry;     /* re-run overflowed queries with larger tables (default 1) */

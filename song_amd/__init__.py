@@ -119,7 +119,7 @@ def metric_code(metric):
     return METRIC_NAMES[metric] if isinstance(metric, str) else int(metric)
 
 
-def make_opts(mode=MODE_AUTO, slack=32, hash_log2=0, max_steps=0, retry=1, storage=STORE_F32, rerank=1,
+def make_opts(mode=MODE_AUTO, slack=128, hash_log2=0, max_steps=0, retry=1, storage=STORE_F32, rerank=1,
               visited_bits=0):
     o = SearchOpts()
     lib().song_search_opts_default(ctypes.byref(o))
@@ -169,7 +169,7 @@ class Index:
         _check(lib().song_index_get_storage(self._h, storage, _ptr(codes), _ptr(scales), ctypes.byref(rb)))
         return codes, scales
 
-    def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, storage=STORE_F32,
+    def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=128, hash_log2=0, retry=1, storage=STORE_F32,
                rerank=1, visited_bits=0):
         """Host buffers in and out: ids int64 [nq, k] (-1 padding), dists float32, flags uint32."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -185,7 +185,7 @@ class Index:
         return out, ds, fl
 
     def search_device(self, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
-                      mode=MODE_AUTO, slack=32, hash_log2=0, retry=1, stream=None, storage=STORE_F32, rerank=1,
+                      mode=MODE_AUTO, slack=128, hash_log2=0, retry=1, stream=None, storage=STORE_F32, rerank=1,
                       max_steps=0, entry=None, steps=None, subset=None, visited_bits=0):
         """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k].
 

@@ -78,7 +78,7 @@ extern "C" const char *song_last_error(void) { return song::g_err.c_str(); }
 
 extern "C" void song_search_opts_default(song_search_opts *o) {
     o->mode = SONG_MODE_AUTO;
-    o->slack = 32;
+    o->slack = 128;
     o->hash_log2 = 0;
     o->max_steps = 0;
     o->retry = 1;
