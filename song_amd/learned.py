@@ -277,7 +277,7 @@ class LearnedSearch:
             if verbose:
                 print(f"[learned] epoch {ep} xent={xe:.3f} stop_mse={ms:.3f}", flush=True)
         torch.cuda.synchronize()
-        top2, conf, stop = self.mlp.infer(X[:min(nt, 20000)], nbr=self.nbr)
+        top2, conf, _ = self.mlp.infer(X[:min(nt, 20000)], nbr=self.nbr)
         acc = (top2[:, 0] == labels[:top2.shape[0]]).float().mean().item()
         self.log = {"kmeans_s": t1 - t0, "traces_s": t2 - t1, "train_s": time.time() - t2, "loss": hist,
                     "train_top1_acc": acc, "mean_conf": conf.mean().item(),
