@@ -109,9 +109,9 @@ int song_index_get_storage(const song_index *index, int storage, void *codes, fl
 /*
  * Batched search with host buffers.  queries: nq x dim float32 (dense,
  * organize_point order, data.h:106-114).  Each query returns the first k
- * (k <= pq) entries of its pq-sized result list, ascending by (dist, id):
- * out_ids nq x k (UINT64_MAX padding when fewer were found), out_dist nq x k
- * float32 (nullable), out_flags nq (nullable).
+ * entries of its pq-sized result list, ascending by (dist, id): out_ids nq x k
+ * (UINT64_MAX padding when fewer were found, and in columns pq..k-1 when k > pq),
+ * out_dist nq x k float32 (+inf padding; nullable), out_flags nq (nullable).
  */
 int song_search(song_index *index, const float *queries, uint64_t nq, int pq, int k,
                 uint64_t *out_ids, float *out_dist, uint32_t *out_flags, const song_search_opts *opts);

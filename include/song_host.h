@@ -10,7 +10,7 @@
  *                            bfsg.graph; runs on several threads.
  *   song_libsvm_parse     <- Parser (parser.h:12-64): one-based "label idx:val ..." lines,
  *                            values read as %lf and cast to float.
- *   song_bfsg_write/read  <- Data::dump/load (data.h:164-175) and the graph dump/load
+ *   song_file_write/read  <- Data::dump/load (data.h:164-175) and the graph dump/load
  *                            (graph.h:259-271): raw little-endian arrays, no header.
  */
 #ifndef SONG_HOST_H

@@ -115,6 +115,41 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def _check_tensor(t, name, dtypes, rows, cols=None, device=None, ndim=None):
+    """Argument check before a raw pointer goes to the library: dtype, contiguity, device and
+    enough rows (and columns) for the kernel's writes and reads."""
+    import torch
+
+    if t is None:
+        return
+    if not isinstance(t, torch.Tensor):
+        raise SongError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype not in dtypes:
+        raise SongError(f"{name}: dtype {t.dtype} not in {[str(d) for d in dtypes]}")
+    if not t.is_cuda or (device is not None and t.device != device):
+        raise SongError(f"{name}: must be on {device or 'a cuda device'}, is on {t.device}")
+    if not t.is_contiguous():
+        raise SongError(f"{name}: must be contiguous")
+    if ndim is not None and t.dim() not in ndim:
+        raise SongError(f"{name}: {t.dim()} dimensions, expected {ndim}")
+    if t.shape[0] < rows:
+        raise SongError(f"{name}: {t.shape[0]} rows, need at least {rows}")
+    if cols is not None and (t.dim() < 2 or t.shape[1] < cols):
+        raise SongError(f"{name}: shape {tuple(t.shape)}, need at least {cols} columns")
+
+
+def _int32s():
+    import torch
+
+    return (torch.int32, torch.uint32)
+
+
+def _f32():
+    import torch
+
+    return (torch.float32,)
+
+
 def metric_code(metric):
     return METRIC_NAMES[metric] if isinstance(metric, str) else int(metric)
 
@@ -195,6 +230,21 @@ class Index:
         import torch
 
         nq = queries.shape[0]
+        dev = queries.device
+        _check_tensor(queries, "queries", _f32(), nq, self.dim, ndim=(2,))
+        if queries.shape[1] != self.dim:
+            raise SongError(f"queries: {queries.shape[1]} columns, the index has dim {self.dim}")
+        _check_tensor(ids, "ids", _int32s(), nq, k, dev, ndim=(2,))
+        _check_tensor(dists, "dists", _f32(), nq, ids.shape[1], dev, ndim=(2,))
+        _check_tensor(flags, "flags", _int32s(), nq, None, dev, ndim=(1,))
+        _check_tensor(stats, "stats", _int32s(), nq, 8, dev, ndim=(2,))
+        for name, t in (("pq_arr", pq_arr), ("k_arr", k_arr), ("steps", steps)):
+            _check_tensor(t, name, _int32s(), nq, None, dev, ndim=(1,))
+        _check_tensor(entry, "entry", _int32s(), nq, None, dev, ndim=(1, 2))
+        if subset is not None:
+            _check_tensor(subset, "subset", _int32s(), 0, None, dev, ndim=(1,))
+            if subset.numel() and (int(subset.min().item()) < 0 or int(subset.max().item()) >= nq):
+                raise SongError(f"subset: query indices outside [0, {nq})")
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
@@ -294,6 +344,18 @@ class Batch:
         graph=True captures the run once per argument set and replays it."""
         import torch
 
+        nq = queries.shape[0]
+        dev = queries.device
+        _check_tensor(queries, "queries", _f32(), nq, self.index.dim, ndim=(2,))
+        if queries.shape[1] != self.index.dim:
+            raise SongError(f"queries: {queries.shape[1]} columns, the index has dim {self.index.dim}")
+        _check_tensor(ids, "ids", _int32s(), nq, k, dev, ndim=(2,))
+        _check_tensor(flags, "flags", _int32s(), nq, None, dev, ndim=(1,))
+        _check_tensor(dists, "dists", _f32(), nq, ids.shape[1], dev, ndim=(2,))
+        _check_tensor(stats, "stats", _int32s(), nq, 8, dev, ndim=(2,))
+        for name, t in (("pq_arr", pq_arr), ("k_arr", k_arr), ("steps", steps)):
+            _check_tensor(t, name, _int32s(), nq, None, dev, ndim=(1,))
+        _check_tensor(entry, "entry", _int32s(), nq, None, dev, ndim=(1, 2))
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         if pq is None:

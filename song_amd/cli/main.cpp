@@ -1,10 +1,10 @@
-// `song` command line, drop-in for the reference binaries (main.cu:158-205, main.cc:49-105):
+// `song` command line, drop-in for the reference binaries (main.cu:53-100, main.cc:49-105):
 //   song <build/test> <build_data> <query_data> <search_top_k> <row> <dim> <return_top_k> <l2/ip/cos>
 // build: parse the one-based LIBSVM rows, build the graph (byte-identical to the reference
 //        insertion build), write bfsg.data and bfsg.graph to the working directory.
 // test:  load bfsg.data / bfsg.graph, search every query of the LIBSVM file in one GPU
 //        batch, print the first <return_top_k> ids of each result, "%zu " each, one line
-//        per query (main.cu:133-138).
+//        per query (main.cu:28-33).
 // The search budget pq is SONG_PQ when fill_parameters.sh compiled one in (the reference
 // GPU binary's TOPK), otherwise <search_top_k> (the reference CPU binary's k).
 #include <cstdio>
@@ -49,9 +49,12 @@ static int run(const std::string &mode, char **argv, size_t row, int dim, int pq
         fprintf(stderr, "File not found at (%s)\n", argv[3]);
         return 1;
     }
+    // records as fgets reads them (parser.h:57): every '\n', plus a last line without one
     size_t lines = 0;
-    for (int c; (c = fgetc(fp)) != EOF;)
+    int last = '\n';
+    for (int c; (c = fgetc(fp)) != EOF; last = c)
         if (c == '\n') ++lines;
+    if (last != '\n') ++lines;
     fclose(fp);
     std::vector<float> dense(std::max<size_t>(lines, 1) * (size_t)dim);
     uint64_t nq = 0;

@@ -556,7 +556,9 @@ extern "C" int song_search(song_index *ix, const float *queries, uint64_t nq, in
     if (nq == 0) return SONG_OK;
     if (!queries || !out_ids) return fail(SONG_E_ARG, "queries/out_ids are null");
     if (pq < 1 || k < 1) return fail(SONG_E_ARG, "pq=%d k=%d", pq, k);
-    k = std::min(k, pq);
+    // rows keep the caller's stride k; a search returns at most pq entries, so columns
+    // [pq, k) are padding (UINT64_MAX, +inf) like a list shorter than k
+    const int kk = std::min(k, pq);
     float *d_q = nullptr, *d_d = nullptr;
     uint32_t *d_i = nullptr, *d_f = nullptr;
     int rc = SONG_OK;
@@ -567,15 +569,17 @@ extern "C" int song_search(song_index *ix, const float *queries, uint64_t nq, in
     }
     if (rc == SONG_OK && hipMemcpy(d_q, queries, qb, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(SONG_E_HIP, "upload queries");
-    if (rc == SONG_OK) rc = song_search_device(ix, d_q, nq, pq, k, nullptr, nullptr, d_i, d_d, k, d_f, nullptr, opts, nullptr);
+    if (rc == SONG_OK) rc = song_search_device(ix, d_q, nq, pq, kk, nullptr, nullptr, d_i, d_d, k, d_f, nullptr, opts, nullptr);
     if (rc == SONG_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(SONG_E_HIP, "search sync");
     if (rc == SONG_OK) {
         std::vector<uint32_t> ids(nq * (size_t)k);
         if (hipMemcpy(ids.data(), d_i, ib, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(SONG_E_HIP, "download ids");
         for (size_t i = 0; rc == SONG_OK && i < ids.size(); ++i)
-            out_ids[i] = ids[i] == 0xFFFFFFFFu ? UINT64_MAX : (uint64_t)ids[i];
+            out_ids[i] = (ids[i] == 0xFFFFFFFFu || (int)(i % (size_t)k) >= kk) ? UINT64_MAX : (uint64_t)ids[i];
         if (rc == SONG_OK && out_dist && hipMemcpy(out_dist, d_d, ib, hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(SONG_E_HIP, "download dists");
+        for (size_t i = 0; rc == SONG_OK && out_dist && kk < k && i < ids.size(); ++i)
+            if ((int)(i % (size_t)k) >= kk) out_dist[i] = INFINITY;
         if (rc == SONG_OK && out_flags &&
             hipMemcpy(out_flags, d_f, nq * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(SONG_E_HIP, "download flags");

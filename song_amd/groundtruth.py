@@ -41,19 +41,37 @@ def knn_fp64(base: np.ndarray, queries: np.ndarray, k: int, metric: str = "l2", 
             if best_d is not None:
                 d = torch.cat([best_d, d], 1)
                 ids = torch.cat([best_i, ids], 1)
-            kk = min(k, d.shape[1])
-            # (distance, id) order: stable sort by id first is implied by candidate layout
-            # (earlier chunks hold smaller ids); take a margin then sort lexicographically
-            top = torch.topk(d, min(d.shape[1], kk + 32), dim=1, largest=False, sorted=True)
-            td, ti = top.values, torch.gather(ids, 1, top.indices)
-            order = _lexsort(td, ti)
-            best_d = torch.gather(td, 1, order)[:, :kk]
-            best_i = torch.gather(ti, 1, order)[:, :kk]
+            best_d, best_i = _smallest_k(d, ids, min(k, d.shape[1]))
         out_d[qs:qs + q.shape[0]] = best_d.cpu().numpy()
         out_i[qs:qs + q.shape[0]] = best_i.cpu().numpy()
     if metric == "l2":
         np.maximum(out_d, 0.0, out=out_d)
     return out_i, out_d
+
+
+def _smallest_k(d, ids, kk, margin=32):
+    """The kk smallest (distance, id) pairs of each row, ascending.  topk over a margin of
+    extra entries, then a lexicographic sort; a row whose margin is still tied with its kk-th
+    distance may have lost tie members with smaller ids to topk's arbitrary choice, so that row
+    is redone over every entry at or below its kk-th distance."""
+    import torch
+
+    m = min(d.shape[1], kk + margin)
+    top = torch.topk(d, m, dim=1, largest=False, sorted=True)
+    td, ti = top.values, torch.gather(ids, 1, top.indices)
+    order = _lexsort(td, ti)
+    best_d = torch.gather(td, 1, order)[:, :kk].contiguous()
+    best_i = torch.gather(ti, 1, order)[:, :kk].contiguous()
+    if m < d.shape[1]:
+        cut = td[:, kk - 1]
+        rows = torch.nonzero(td[:, m - 1] <= cut).flatten().tolist()
+        for r in rows:
+            sel = torch.nonzero(d[r] <= cut[r]).flatten()
+            rd, ri = d[r, sel][None], ids[r, sel][None]
+            o = _lexsort(rd, ri)
+            best_d[r] = torch.gather(rd, 1, o)[0, :kk]
+            best_i[r] = torch.gather(ri, 1, o)[0, :kk]
+    return best_d, best_i
 
 
 def _lexsort(d, i):

@@ -38,7 +38,7 @@ import time
 
 import numpy as np
 
-from . import SongError, lib, _check, Batch, MODE_FP32, tier
+from . import FLAG_QUEUE_OVF, FLAG_VISITED_OVF, MODE_FP32, Batch, SongError, _check, lib, tier
 
 
 def _mlp_lib():
@@ -346,28 +346,47 @@ class LearnedSearch:
                 "settle_max": int(settle.max()), "expansions_mean": round(float(st[:, 0].float().mean().item()), 1)}
 
     def search(self, q_dev, pq, k, ids, plan=None, pq_fallback=None, conf_threshold=0.9, step_scale=1.5, step_add=8,
-               storage=0, dists=None, stats=None, fallback_cap=None):
+               storage=0, dists=None, stats=None, fallback_cap=None, flags=None):
         """Learned search of a batch into ids (int32 cuda [nq, >=k]); returns the plan.
-        fallback_cap: expansion cap of the fallback queries (None: uncapped plain search)."""
+        fallback_cap: expansion cap of the fallback queries (None: uncapped plain search).
+        flags (int32 cuda [nq], optional) receives each query's flags.  The two groups run
+        side by side without host waits; queries whose visited table or queue overflowed are
+        then searched again with the library's re-run (larger tables), so no result is
+        truncated."""
         if plan is None:
             plan = self.plan(q_dev, conf_threshold, step_scale, step_add)
         import torch
 
         entry, steps, confident, fallback = plan
+        nq = q_dev.shape[0]
+        if flags is None:
+            flags = torch.zeros(nq, dtype=torch.int32, device=self.dev)
         main = torch.cuda.current_stream()
+        fsteps = None if fallback_cap is None else torch.full_like(steps, int(fallback_cap))
         if fallback.numel():
             # the fallback group runs beside the confident one on a second stream
             side = self._side_stream()
             side.wait_stream(main)
             with torch.cuda.stream(side):
-                fsteps = None if fallback_cap is None else torch.full_like(steps, int(fallback_cap))
-                self.index.search_device(q_dev, pq_fallback or pq, k, ids, dists=dists, stats=stats,
+                self.index.search_device(q_dev, pq_fallback or pq, k, ids, dists=dists, flags=flags, stats=stats,
                                          subset=fallback, storage=storage, retry=0, steps=fsteps)
         if confident.numel():
-            self.index.search_device(q_dev, pq, k, ids, dists=dists, stats=stats, entry=entry, steps=steps,
-                                     subset=confident, storage=storage, retry=0)
+            self.index.search_device(q_dev, pq, k, ids, dists=dists, flags=flags, stats=stats, entry=entry,
+                                     steps=steps, subset=confident, storage=storage, retry=0)
         if fallback.numel():
             main.wait_stream(self._side_stream())
+        ovf = (flags & (FLAG_VISITED_OVF | FLAG_QUEUE_OVF)) != 0
+        if bool(ovf.any().item()):
+            conf_mask = torch.zeros(nq, dtype=torch.bool, device=self.dev)
+            conf_mask[confident.long()] = True
+            redo_f = torch.nonzero(ovf & ~conf_mask).flatten().to(torch.int32)
+            redo_c = torch.nonzero(ovf & conf_mask).flatten().to(torch.int32)
+            if redo_f.numel():
+                self.index.search_device(q_dev, pq_fallback or pq, k, ids, dists=dists, flags=flags, stats=stats,
+                                         subset=redo_f, storage=storage, retry=1, steps=fsteps)
+            if redo_c.numel():
+                self.index.search_device(q_dev, pq, k, ids, dists=dists, flags=flags, stats=stats, entry=entry,
+                                         steps=steps, subset=redo_c, storage=storage, retry=1)
         return plan
 
     def _side_stream(self):

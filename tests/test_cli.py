@@ -60,3 +60,24 @@ def test_usage_and_errors(tmp_path):
     r = subprocess.run([SONG, "build", "bad.svm", "0", "0", "10", "4", "0", "l2"], cwd=tmp_path,
                        capture_output=True, text=True)
     assert r.returncode == 1 and "feature index" in r.stderr
+
+
+def test_query_file_without_trailing_newline_is_read_whole(tmp_path):
+    """`song test` counts query records as fgets reads them (parser.h:57): a last line without
+    '\\n' is a record.  Without a GPU the run gets past the parse and stops at the index upload."""
+    meta, base, query, _, _ = golden_cases.load("tiny_l2_500")
+    n, dim = base.shape
+    datasets.write_libsvm(str(tmp_path / "base.svm"), base)
+    r = subprocess.run([SONG, "build", "base.svm", "0", "0", str(n), str(dim), "0", "l2"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    datasets.write_libsvm(str(tmp_path / "q.svm"), query[:3])
+    text = (tmp_path / "q.svm").read_text()
+    (tmp_path / "q.svm").write_text(text.rstrip("\n"))
+    r = subprocess.run([SONG, "test", "0", "q.svm", "20", str(n), str(dim), "5", "l2"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert "more than" not in r.stderr and "parse" not in r.stderr, r.stderr
+    if r.returncode != 0:
+        assert "GPU" in r.stderr or "index" in r.stderr, r.stderr
+    else:
+        assert len(r.stdout.strip("\n").split("\n")) == 3

@@ -298,3 +298,59 @@ def test_batch_fp32_integer_mixture():
     b = song_amd.Batch(ix, [tier(64), tier(512, waves_per_group=4)], len(query), mode=song_amd.MODE_FP32)
     got = _batch_run(b, q, torch.from_numpy(k).to(dev), torch.from_numpy(pq).to(dev))
     _same(got, want, "mixture fp32")
+
+
+# ---------------------------------------------------------------- inner product (the 2M x 96 generator)
+
+@pytest.fixture(scope="module")
+def ip96():
+    """Unit-norm Gaussian rows, 96-d, inner product (the batching workload's generator) on the
+    golden ip_10k case: its graph, its 300 reference-searched queries at pq 64 (k = 64), and
+    1,700 more queries of the same generator with uneven (k, pq)."""
+    import torch
+
+    meta, base, _, ref_ids, graph = golden_cases.load("ip_10k")
+    graph = song_amd.build_graph(base, "ip", threads=8) if graph is None else graph
+    assert golden_cases.sha256(graph) == meta["graph_sha256"]
+    _, query = datasets.normalized_ip(n_base=meta["kwargs"]["n_base"], n_query=2000, dim=96, seed=4)
+    k, pq = datasets.hetero_budgets(len(query), seed=4, ks=(1, 10, 100), pqs=(32, 64, 128, 256, 512))
+    ng = meta["n_query"]
+    pq[:ng], k[:ng] = meta["pq"], meta["pq"]
+    ix = song_amd.Index(base, graph, "ip")
+    dev = _dev()
+    q = torch.from_numpy(np.ascontiguousarray(query)).to(dev)
+    return dict(meta=meta, base=base, ix=ix, q=q, k=k, pq=pq, ref_ids=ref_ids, ng=ng,
+                k_d=torch.from_numpy(k).to(dev), pq_d=torch.from_numpy(pq).to(dev))
+
+
+IP_CONFIGS = {
+    "tier_per_pq": [tier(p) for p in (32, 64, 128, 256, 512)],
+    "two_tiers_2waves_spill": [tier(64, waves_per_group=2, hash_log2=9),
+                               tier(512, waves_per_group=4, slack=8, groups_per_cu=1)],
+}
+
+
+@pytest.mark.parametrize("name", sorted(IP_CONFIGS))
+def test_ip_batch_equals_plain_and_reference(ip96, name):
+    """search_persistent on inner product (exact mode): every query equals the plain search at its
+    (k, pq), and the golden queries equal the reference song.cpu's ids."""
+    print(f"[batch-ip] config {name}", flush=True)
+    c = ip96
+    want = _plain(c["ix"], c["q"], c["k"], c["pq"], song_amd.MODE_EXACT)
+    b = song_amd.Batch(c["ix"], IP_CONFIGS[name], c["q"].shape[0], mode=song_amd.MODE_EXACT)
+    for graph in (False, True):
+        got = _batch_run(b, c["q"], c["k_d"], c["pq_d"], graph=graph)
+        _same(got, want, f"{name} graph={graph}")
+    ng, pq = c["ng"], c["meta"]["pq"]
+    ids = got[0][:ng, :pq].cpu().numpy().astype(np.int64)
+    assert np.array_equal(ids, c["ref_ids"][:, :pq])
+    b.close()
+
+
+def test_ip_batch_fp32_mode_equals_plain_fp32(ip96):
+    """The FP32-accumulation persistent kernel on inner product equals the FP32 plain search."""
+    c = ip96
+    want = _plain(c["ix"], c["q"], c["k"], c["pq"], song_amd.MODE_FP32)
+    b = song_amd.Batch(c["ix"], IP_CONFIGS["tier_per_pq"], c["q"].shape[0], mode=song_amd.MODE_FP32)
+    _same(_batch_run(b, c["q"], c["k_d"], c["pq_d"]), want, "ip fp32")
+    b.close()

@@ -1,4 +1,4 @@
-"""`song test` on the GPU: the printed ids ("%zu " each, one line per query, main.cu:133-138)
+"""`song test` on the GPU: the printed ids ("%zu " each, one line per query, main.cu:28-33)
 equal the reference song.cpu's ids for the same files."""
 import os
 import shutil
@@ -60,3 +60,23 @@ def test_test_query_script_display(tmp_path):
     graph = pyoracle.build(base, "l2")
     o_ids = pyoracle.search_batch(base, graph, "l2", query, 100, threads=4)[0]
     assert np.array_equal(np.array(got), np.asarray(o_ids)[:, :5].astype(np.int64))
+
+
+def test_query_file_without_trailing_newline(tmp_path):
+    """The last query line has no '\\n': it is still searched (parser.h:57 reads it with fgets)."""
+    meta, base, query, ids, _ = golden_cases.load("tiny_cos_500")
+    n, dim = base.shape
+    datasets.write_libsvm(str(tmp_path / "base.svm"), base)
+    datasets.write_libsvm(str(tmp_path / "query.svm"), query)
+    (tmp_path / "query.svm").write_text((tmp_path / "query.svm").read_text().rstrip("\n"))
+    m, pq = meta["metric"], meta["pq"]
+    r = subprocess.run([SONG, "build", "base.svm", "0", "0", str(n), str(dim), "0", m], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([SONG, "test", "0", "query.svm", str(pq), str(n), str(dim), str(pq), m], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.split("\n")
+    assert len(lines) == len(query) + 1
+    for i, line in enumerate(lines[:-1]):
+        assert line == "".join(f"{v} " for v in ids[i] if v >= 0), i

@@ -120,11 +120,15 @@ def test_training_learns_nearest_centre_classes():
     assert last < 0.5 * first and acc > 0.9
 
 
-@pytest.fixture(scope="module")
-def mix50k():
-    base, _ = datasets.mixture_l2(n_base=50000, n_query=1, dim=32, n_centers=100, seed=12)
-    train = datasets.mixture_l2_queries(20000, dim=32, n_centers=100, seed=12, stream=1)
-    test = datasets.mixture_l2_queries(1000, dim=32, n_centers=100, seed=12, stream=2)
+# 32-d rows, and 128-d rows of the 1M x 128 workload's generator (integer-valued Gaussian mixture)
+@pytest.fixture(scope="module", params=[32, 128], ids=["dim32", "dim128"])
+def mix50k(request):
+    dim = request.param
+    centres = 100 if dim == 32 else 200
+    print(f"[learned] mixture 50k x {dim}, {centres} centres", flush=True)
+    base, _ = datasets.mixture_l2(n_base=50000, n_query=1, dim=dim, n_centers=centres, seed=12)
+    train = datasets.mixture_l2_queries(20000, dim=dim, n_centers=centres, seed=12, stream=1)
+    test = datasets.mixture_l2_queries(1000, dim=dim, n_centers=centres, seed=12, stream=2)
     graph = song_amd.build_graph(base, "l2", threads=8)
     ix = song_amd.Index(base, graph, "l2")
     return base, graph, ix, train, test
