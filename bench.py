@@ -10,7 +10,8 @@ queries, index and outputs resident in HBM.
 
 Forms timed:
   plain          the bit-exact search from vertex 0, one launch per batch (one
-                 workgroup per query); the smallest pq of the sweep whose recall@10
+                 workgroup per query); the smallest pq (a coarse grid, then a
+                 bisection in steps of 16) whose recall@10 on the selection split
                  (tie-aware, against a float64 brute force on the GPU) reaches the
                  target.  The ratio's denominator.
   plain batched  the same pq through the batching form (autotuned persistent
@@ -21,11 +22,19 @@ Forms timed:
                  to the plain full-budget search; every query in one batched search
                  (per-query pq / entries / caps made on the GPU, autotuned tiers),
                  the whole pipeline -- features, MLP, plan, search -- one graph
-                 replay per step.  `value` is its QPS at the fastest setting whose
-                 recall reaches the target.
+                 replay per step.  `value` is its QPS at the setting chosen on the
+                 selection split, with the fallback uncapped; the same setting with
+                 the fallback under a training-calibrated expansion cap is reported
+                 beside it.
+Every setting (the plain pq, the learned pq and stop scale, the autotuned tiers) is chosen
+on a held-out selection split -- 10,000 more queries of the same mixture on their own
+random stream -- and then timed and scored on the test queries.
+
 `--workload c4` measures the batching form on its own workload instead (2M x 96
 inner product, 100,000 queries with uneven k and pq); `--workload c3` the precision
-form (factor-model 960-d L2: recall deltas and distance errors of BF16 / FP8 rows).
+form (factor-model 960-d L2: recall deltas and distance errors of BF16 / FP8 rows);
+`--workload c5` the C2 measurement on the same generator at 10,000,000 rows and
+100,000 queries.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -111,35 +120,47 @@ def load_traffic(pq):
     return t if t.get("pq") == pq else None
 
 
-def cpu_baseline(base, graph, metric, query, groups, seconds, check=None):
-    """Oracle restatement of graph.h:169-216 on the host cores over a bounded sample.
-    groups: list of (pq, query index array); the sample takes the same share of each."""
+def cpu_baseline(base, graph, metric, query, groups, seconds, check=None, repeats=3, one_core_queries=1000):
+    """Oracle restatement of graph.h:169-216 on the host cores over a bounded sample, the median of
+    `repeats` timed passes; plus a single-core figure over `one_core_queries` queries (median of
+    the same repeats).  groups: list of (pq, query index array); the sample takes the same share of
+    each.  Threads: min(16, cores) -- one GPU's share of the host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    threads = min(16, os.cpu_count() or 1)
-    probe = []
-    for pq, idx in groups:
-        tp = time.perf_counter()
-        pyoracle.search_batch(base, graph, metric, query[idx[:8]], pq, threads=1)
-        probe.append((time.perf_counter() - tp) / min(8, len(idx)))
+    nproc = os.cpu_count() or 1
+    threads = min(16, nproc)
     total_q = sum(len(idx) for _, idx in groups)
-    mean_t = sum(p * len(idx) for p, (_, idx) in zip(probe, groups)) / total_q
-    frac = min(1.0, seconds * threads / max(mean_t * total_q, 1e-9))
-    done, tcpu, same = 0, 0.0, True
-    for pq, idx in groups:
-        take = idx[:max(threads, int(len(idx) * frac))]
-        tc = time.perf_counter()
-        cids, _ = pyoracle.search_batch(base, graph, metric, query[take], pq, threads=threads)
-        tcpu += time.perf_counter() - tc
-        done += len(take)
-        if check is not None:
-            same &= check(pq, take, cids)
-    return {"value": round(done / tcpu, 1), "unit": "queries/s", "cores": threads, "kind": "port",
+
+    def one_pass(take_frac, nthreads, check_ids):
+        done, tcpu, same = 0, 0.0, True
+        for pq, idx in groups:
+            take = idx[:max(nthreads, int(round(len(idx) * take_frac)))]
+            tc = time.perf_counter()
+            cids, _ = pyoracle.search_batch(base, graph, metric, query[take], pq, threads=nthreads)
+            tcpu += time.perf_counter() - tc
+            done += len(take)
+            if check_ids and check is not None:
+                same &= check(pq, take, cids)
+        return done, tcpu, same
+
+    # single core: the same share of every group, about one_core_queries queries in all
+    f1 = min(1.0, one_core_queries / max(total_q, 1))
+    one = [one_pass(f1, 1, False) for _ in range(repeats)]
+    q1 = one[0][0]
+    rate1 = float(np.median([d / t for d, t, _ in one]))
+    # all threads: as much of the batch as fits `seconds` of CPU time at the single-core rate
+    frac = min(1.0, seconds * threads * rate1 / max(total_q, 1))
+    runs = [one_pass(frac, threads, i == 0) for i in range(repeats)]
+    done = runs[0][0]
+    rates = [d / t for d, t, _ in runs]
+    return {"value": round(float(np.median(rates)), 1), "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"{done} of the {total_q} queries (same share of every pq group), oracle restatement of "
-                      f"graph.h:169-216 on {threads} threads, static split; single-thread probe "
-                      f"{1.0 / mean_t:.1f} q/s", "cpu_model": cpu_model(),
-            "ids_equal_gpu": bool(same) if check is not None else None}
+                      f"graph.h:169-216 on {threads} threads (static split), median of {repeats} passes "
+                      f"(passes: {', '.join(f'{r:.0f}' for r in rates)} q/s); single core: {q1} queries, "
+                      f"median of {repeats} passes",
+            "one_core_qps": round(rate1, 1), "repeats": repeats, "nproc": nproc, "cpu_model": cpu_model(),
+            "ids_equal_gpu": bool(runs[0][2]) if check is not None else None}
 
 
 def bench_c2(args, torch, song_amd):
@@ -147,7 +168,13 @@ def bench_c2(args, torch, song_amd):
 
     dev = torch.device("cuda:0")
     t0 = time.time()
-    base, query, graph, metric, bst = workloads.load("c2_1m", verbose=True)
+    if args.workload == "c5":
+        # the same generator at 10M rows and 100k queries (the learned form's large workload)
+        base, query = datasets.mixture_l2(n_base=args.c5_n, n_query=args.c5_nq)
+        graph, bst = workloads.graph_for(base, "l2", verbose=True)
+        metric = "l2"
+    else:
+        base, query, graph, metric, bst = workloads.load("c2_1m", verbose=True)
     log(f"workload ready in {time.time() - t0:.1f}s (graph cached={bst.get('cached')})")
     ix = song_amd.Index(base, graph, metric)
     nq, dim = query.shape
@@ -162,63 +189,91 @@ def bench_c2(args, torch, song_amd):
         log(f"calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
         del src, dst
 
-    gt_d = None
+    # held-out selection split: queries from the same mixture as the test queries, on their own
+    # stream (the learned form trains on stream 1); every setting is chosen on this split and then
+    # timed and scored on the 10,000 test queries
+    selq = datasets.mixture_l2_queries(args.sel_queries, seed=2, stream=3)
+    gt_d = gs_d = None
     if not args.no_recall:
         t1 = time.time()
         _, gt_d = groundtruth.knn_fp64(base, query, k, metric)
-        log(f"fp64 ground truth in {time.time() - t1:.1f}s")
+        _, gs_d = groundtruth.knn_fp64(base, selq, k, metric)
+        log(f"fp64 ground truth (test and selection queries) in {time.time() - t1:.1f}s")
 
-    def recall(ids_t):
-        if gt_d is None:
+    def recall(ids_t, gd, qs):
+        if gd is None:
             return float("nan")
-        return groundtruth.recall_at_k(ids_t.cpu().numpy().astype(np.int64), gt_d, k, base, query, metric)
+        return groundtruth.recall_at_k(ids_t.cpu().numpy().astype(np.int64), gd, k, base, qs, metric)
 
     q = torch.from_numpy(query).to(dev)
+    qs = torch.from_numpy(selq).to(dev)
+    ns = qs.shape[0]
     flags = torch.empty(nq, dtype=torch.int32, device=dev)
     stats = torch.empty((nq, 8), dtype=torch.int32, device=dev)
     ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    s_ids = torch.empty((ns, k), dtype=torch.int32, device=dev)
+    s_flags = torch.empty(ns, dtype=torch.int32, device=dev)
     # integer-valued data and queries: FP32 accumulation is exact (same ids as the exact mode)
     mode = song_amd.resolve_mode(ix, q)
 
-    def plain(pq, st=None, retry=0):
-        ix.search_device(q, pq, k, ids, None, flags, st, mode=mode, retry=retry)
+    def plain(qd, out, fl, pq, st=None, retry=0):
+        ix.search_device(qd, pq, k, out, None, fl, st, mode=mode, retry=retry)
 
     # ---------------------------------------------------------------- plain form
     sweep = []
-    chosen = None
-    pqs = [args.pq] if args.pq else [int(x) for x in args.pqs.split(",")]
-    for pq in pqs:
-        plain(pq, stats, retry=1)
+
+    def try_pq(pq):
+        plain(qs, s_ids, s_flags, pq, retry=1)
         torch.cuda.synchronize()
-        r = recall(ids)
-        st = stats.cpu().numpy()
-        ms = timed(lambda: plain(pq), 3, torch)
-        sweep.append({"pq": pq, "recall@10": round(r, 4), "qps": round(nq / (ms / 1e3), 1),
-                      "mean_expansions": round(float(st[:, 0].mean()), 1),
-                      "mean_distances": round(float(st[:, 1].mean()), 1),
-                      "prefetch_hit": round(float(st[:, 3].sum() / max(1, st[:, 0].sum())), 4)})
-        log(f"plain pq={pq} recall@{k}={r:.4f} {ms:.2f} ms ({nq / (ms / 1e3):.0f} q/s) "
-            f"exp={st[:, 0].mean():.1f} dist={st[:, 1].mean():.1f}")
-        if chosen is None and (r >= args.target_recall or args.no_recall):
-            chosen = (pq, r, st)
-            if not args.pq:
-                break
-    if chosen is None:
-        pq, r, st = pqs[-1], sweep[-1]["recall@10"], stats.cpu().numpy()
-        log(f"plain form did not reach the target recall; timing pq={pq}")
+        r = recall(s_ids, gs_d, selq)
+        ms = timed(lambda: plain(qs, s_ids, s_flags, pq), 3, torch)
+        sweep.append({"pq": pq, "recall@10_selection": round(r, 4), "qps_selection": round(ns / (ms / 1e3), 1)})
+        log(f"plain pq={pq} selection recall@{k}={r:.4f} ({ns / (ms / 1e3):.0f} q/s)")
+        return r >= args.target_recall
+
+    if args.pq:
+        pq = args.pq
+    elif args.no_recall:
+        ap_error("--no-recall needs --pq")
     else:
-        pq, r, st = chosen
-    plain(pq, None, retry=1)
+        # coarse grid up to the first budget reaching the target on the selection split, then a
+        # bisection on multiples of --pq-step between the last miss and that budget
+        grid = [int(x) for x in args.pqs.split(",")]
+        lo, hi = 0, None
+        for g in grid:
+            if try_pq(g):
+                hi = g
+                break
+            lo = g
+        if hi is None:
+            pq = grid[-1]
+            log(f"plain form did not reach the target recall on the selection split; timing pq={pq}")
+        else:
+            step = args.pq_step
+            while hi - lo > step:
+                mid = lo + max(step, ((hi - lo) // 2) // step * step)
+                if mid >= hi:
+                    break
+                if try_pq(mid):
+                    hi = mid
+                else:
+                    lo = mid
+            pq = hi
+    plain(q, ids, flags, pq, stats, retry=1)
     torch.cuda.synchronize()
+    r = recall(ids, gt_d, query)
+    st = stats.cpu().numpy()
     plain_ids = ids.clone()
     retry = 1 if int((flags.cpu().numpy() & 3).sum()) > 0 else 0
 
-    ms_plain, wall_plain = timed_steps(lambda: plain(pq, retry=retry), args.steps, args.warmup, torch, stream)
+    ms_plain, wall_plain = timed_steps(lambda: plain(q, ids, flags, pq, retry=retry), args.steps, args.warmup,
+                                       torch, stream)
     qps_plain = nq / (ms_plain / 1e3)
     plain_info = {"pq": pq, "recall@10": round(r, 4), "qps": round(qps_plain, 1), "ms_per_step": round(ms_plain, 4),
                   "wall_ms_per_step": round(wall_plain, 4), "mean_expansions": round(float(st[:, 0].mean()), 1),
-                  "mean_distances": round(float(st[:, 1].mean()), 1), "arithmetic": "fp32 (exact on integer data)"}
-    log(f"plain form: {qps_plain:.0f} q/s at pq={pq} recall={r:.4f}")
+                  "mean_distances": round(float(st[:, 1].mean()), 1), "arithmetic": "fp32 (exact on integer data)",
+                  "pq_chosen_on": "selection split" if not args.pq else "--pq"}
+    log(f"plain form: {qps_plain:.0f} q/s at pq={pq}, test recall={r:.4f}")
     alg_plain = alg_bytes(st, dim, dim * 4, nq)
     # host buffers in and out (song_search: device buffers allocated per call, queries up,
     # ids and distances down over PCIe) -- reported beside the resident-input rate
@@ -229,10 +284,11 @@ def bench_c2(args, torch, song_amd):
     # ---------------------------------------------------------------- plain, batched
     batched_info = None
     if not args.no_batch:
-        pq_all = torch.full((nq,), pq, dtype=torch.int32, device=dev)
-        k_all = torch.full((nq,), k, dtype=torch.int32, device=dev)
+        # tuned on the selection queries, timed on the test queries
+        pq_s = torch.full((ns,), pq, dtype=torch.int32, device=dev)
+        k_s = torch.full((ns,), k, dtype=torch.int32, device=dev)
         tb = time.time()
-        table = autotune.tune(ix, q, pq_all, k_all, mode=mode, reps=3, max_queries_per_tier=nq, k_ld=k)
+        table = autotune.tune(ix, qs, pq_s, k_s, mode=mode, reps=3, max_queries_per_tier=ns, k_ld=k)
         log(f"plain batched: autotuned in {time.time() - tb:.1f}s: {table['tiers']}")
         b = song_amd.Batch(ix, table["tiers"], nq, mode=mode)
         side = torch.cuda.Stream()
@@ -253,106 +309,8 @@ def bench_c2(args, torch, song_amd):
     # ---------------------------------------------------------------- learned + BF16, batched
     learned_info = None
     if not args.no_learned and not args.no_recall:
-        from song_amd import learned
-
-        ix.add_storage(song_amd.STORE_BF16)
-        train = datasets.mixture_l2_queries(args.train_queries, seed=2, stream=1)
-        ls = learned.LearnedSearch(ix, base, n_pivots=4096, hidden=256, seed=0, n_nbr=8)
-        t2 = time.time()
-        fit = ls.fit(train, pq_label=256, k=k, epochs=8, batch=1024, lr=2e-3, verbose=False)
-        log(f"learned predictor trained in {time.time() - t2:.1f}s: top1 acc {fit['train_top1_acc']:.3f} "
-            f"mean conf {fit['mean_conf']:.3f} mean settle {fit['mean_settle']:.1f}")
-        fcal, fcap = None, None
-        if args.fallback_quantile > 0:
-            fcal = ls.calibrate_fallback_cap(train, pq, k=k, quantile=args.fallback_quantile)
-            fcap = fcal["cap"]
-            log(f"fallback cap {fcap} expansions (settle quantile {args.fallback_quantile} over {fcal['queries']} "
-                f"training queries, settle mean {fcal['settle_mean']}, max {fcal['settle_max']})")
-        lst = torch.empty((nq, 8), dtype=torch.int32, device=dev)
-
-        def ltiers(lpq):
-            # the learned queries read BF16 rows (the form's storage); the fallback tier is the plain
-            # search and reads its FP32 rows
-            if lpq >= pq:
-                return None
-            return [song_amd.tier(lpq), song_amd.tier(pq, storage=song_amd.STORE_F32)]
-
-        lsweep = []
-        cands = []  # settings that reach the target recall
-        for lpq in [int(x) for x in args.learned_pqs.split(",")]:
-            for sc in [float(x) for x in args.step_scales.split(",")]:
-                scale = sc if sc > 0 else 1e6
-                lb = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                          storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap,
-                                          tiers=ltiers(lpq))
-                plan = lb.run(q, stats=lst)
-                torch.cuda.synchronize()
-                rl = recall(lb.ids)
-                ms = timed(lambda: lb.run(q), 3, torch)
-                stl = lst.cpu().numpy()
-                fb = float((~plan[3]).float().mean().item())
-                ent = {"pq": lpq, "step_scale": sc, "recall@10": round(rl, 4), "qps": round(nq / (ms / 1e3), 1),
-                       "fallback_frac": round(fb, 4), "mean_expansions": round(float(stl[:, 0].mean()), 1),
-                       "mean_distances": round(float(stl[:, 1].mean()), 1)}
-                lsweep.append(ent)
-                log(f"learned+bf16 pq={lpq} scale={sc} recall={rl:.4f} {ms:.2f} ms ({ent['qps']:.0f} q/s) "
-                    f"fallback={fb:.3f} exp={ent['mean_expansions']}")
-                if rl >= args.target_recall:
-                    cands.append((ent, lpq, scale, stl))
-                lb.batch.close()
-        def final_measure(ent, lpq, scale, stl):
-            # autotuned tiers, the pipeline captured as one graph, timed replays
-            tt = time.time()
-            lb0 = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                       storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=fcap,
-                                       tiers=ltiers(lpq))
-            p = lb0.run(q)
-            table = autotune.tune(ix, q, p[0], lb0.k_arr, tiers=lb0.tiers, mode=song_amd.MODE_FP32,
-                                  storage=song_amd.STORE_BF16, reps=3, max_queries_per_tier=nq, k_ld=k, entry=p[1],
-                                  steps=p[2], freeze=("storage",))
-            torch.cuda.synchronize()
-            eager_ids = lb0.ids.clone()
-            lb0.batch.close()
-            log(f"learned tiers autotuned in {time.time() - tt:.1f}s: {table['tiers']}")
-            lb = learned.LearnedBatch(ls, nq, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
-                                      storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, tiers=table["tiers"],
-                                      fallback_cap=fcap)
-            plan_g = lb.capture(q)
-            lb.replay()
-            torch.cuda.synchronize()
-            same = bool(torch.equal(lb.ids, eager_ids))
-            # the fallback queries against the uncapped plain search of the same queries
-            fbm = ~plan_g[3]
-            n_fb = int(fbm.sum().item())
-            fb_capped = int(((lb.flags[fbm] & song_amd.FLAG_STEP_CAP) != 0).sum().item())
-            fb_equal = int((lb.ids[fbm][:, :k] == plain_ids[fbm][:, :k]).all(1).sum().item())
-            fallback_info = {"queries": n_fb, "cap": fcap, "capped": fb_capped, "top10_ids_equal_plain": fb_equal,
-                             "calibration": fcal}
-            log(f"fallback: {n_fb} queries, cap {fcap}: {fb_capped} capped, {fb_equal} with the plain search's top-{k}")
-            r_final = recall(lb.ids)
-            ms_l, wall_l = timed_steps(lb.replay, args.steps, args.warmup, torch, stream)
-            alg_l = alg_bytes(stl, dim, dim * 2 + 4, nq)
-            fit_out = {kk: vv for kk, vv in fit.items() if kk != "loss"}
-            fit_out["final_loss"] = fit["loss"][-1]
-            info = dict(ent, qps=round(nq / (ms_l / 1e3), 1), ms_per_step=round(ms_l, 4),
-                        wall_ms_per_step=round(wall_l, 4), recall_graph=round(r_final, 4),
-                        graph_ids_equal_eager=same, algorithmic_bytes_per_step=alg_l, tiers=table["tiers"],
-                        fallback=fallback_info, training=fit_out)
-            log(f"learned+bf16 batched pq={lpq} scale={ent['step_scale']}: {info['qps']:.0f} q/s "
-                f"({info['qps'] / qps_plain:.2f}x plain), recall {r_final:.4f}, graph == eager: {same}")
-            lb.batch.close()
-            return info
-
-        # the sweep's eager timings are close; the fastest few go through the full measurement
-        cands.sort(key=lambda c: -c[0]["qps"])
-        finals = [final_measure(*c) for c in cands[:args.learned_finalists]]
-        if finals:
-            learned_info = max(finals, key=lambda f: f["qps"])
-            learned_info["finalists"] = [{kk: f[kk] for kk in ("pq", "step_scale", "qps", "recall@10")} for f in finals]
-            log(f"learned+bf16 batched form: {learned_info['qps']:.0f} q/s ({learned_info['qps'] / qps_plain:.2f}x "
-                f"plain), recall {learned_info['recall_graph']:.4f}")
-        learned_info = learned_info or {"error": "no learned setting reached the target recall"}
-        learned_info["sweep"] = lsweep
+        learned_info = bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_d, pq, k, dim,
+                                     plain_ids, qps_plain, recall, stream)
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
@@ -360,21 +318,23 @@ def bench_c2(args, torch, song_amd):
         pids = plain_ids.cpu().numpy().astype(np.int64)
         cpu = cpu_baseline(base, graph, metric, query, [(pq, np.arange(nq))], args.cpu_seconds,
                            check=lambda p, take, cids: bool(np.array_equal(cids[:, :k].astype(np.int64), pids[take])))
-        log(f"cpu baseline: {cpu['value']} q/s on {cpu['cores']} threads")
+        log(f"cpu baseline: {cpu['value']} q/s on {cpu['cores']} threads, {cpu['one_core_qps']} q/s on one")
 
     have_learned = bool(learned_info and "qps" in learned_info)
     if have_learned:
         value, ms_step, dtype = learned_info["qps"], learned_info["ms_per_step"], "bf16"
         form = ("learned + BF16, batched (MLP start vertices and stop caps, BF16 rows, exact FP32 re-rank; "
-                "confidence fallback to the plain search on FP32 rows under a training-calibrated expansion cap; "
-                "persistent tiers, one graph replay per step)")
+                "confidence fallback to the plain full-budget search on FP32 rows; persistent tiers, one graph "
+                "replay per step; settings chosen on a held-out selection split)")
     else:
         value, ms_step, dtype = round(qps_plain, 1), ms_plain, "fp32"
         form = "plain (bit-exact ids)"
     achieved_plain = alg_plain / (ms_plain / 1e3)
     traffic = load_traffic(pq)
+    n_rows = base.shape[0]
+    size = f"{n_rows // 1_000_000}M" if n_rows % 1_000_000 == 0 else f"{n_rows:,}"
     out = {
-        "metric": "QPS @ recall@10>=0.95, 1M x128 L2, 1 GPU: learned/BF16 form vs plain",
+        "metric": f"QPS @ recall@10>=0.95, {size} x128 L2, 1 GPU: learned/BF16 form vs plain",
         "value": value,
         "unit": "queries/s",
         "n_gpus": 1,
@@ -386,14 +346,15 @@ def bench_c2(args, torch, song_amd):
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic: seeded Gaussian-mixture generator (integer-valued), graph = reference insertion build",
-        "config": {"workload": "1,000,000 x 128 L2 mixture (1,000 centres, sigma 20, clipped/rounded), "
-                               "10,000 queries, top-10",
+        "config": {"workload": f"{n_rows:,} x 128 L2 mixture (1,000 centres, sigma 20, clipped/rounded), "
+                               f"{nq:,} queries, top-10",
                    "form": form, "k": k, "target_recall": args.target_recall,
+                   "selection_queries": ns,
                    "plain": plain_info,
                    "plain_batched": batched_info,
                    "learned_bf16_batched": learned_info,
                    "learned_vs_plain": round(value / qps_plain, 3) if have_learned else None,
-                   "plain_sweep": sweep},
+                   "plain_sweep_selection": sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved_plain / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK, 5),
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
@@ -407,6 +368,141 @@ def bench_c2(args, torch, song_amd):
         "cpu_baseline": cpu,
     }
     return out
+
+
+def ap_error(msg):
+    print(json.dumps({"error": msg}))
+    sys.exit(2)
+
+
+def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_d, pq, k, dim, plain_ids,
+                  qps_plain, recall, stream):
+    """Learned + BF16 batched form.  Trained on 200,000 queries of the mixture (stream 1); the
+    learned budget and stop scale are chosen on the selection split (stream 3), the tiers are
+    autotuned there (the confident tier may take a Bloom filter if the selection recall stays at
+    the target), and the chosen setting is captured and timed on the test queries.  The headline
+    runs the fallback as the plain full-budget search (no expansion cap); the same setting with
+    the fallback under the training-calibrated cap is measured beside it."""
+    from song_amd import autotune, datasets, learned
+
+    dev = q.device
+    nq, ns = q.shape[0], qs.shape[0]
+    ix.add_storage(song_amd.STORE_BF16)
+    train = datasets.mixture_l2_queries(args.train_queries, seed=2, stream=1)
+    ls = learned.LearnedSearch(ix, base, n_pivots=4096, hidden=256, seed=0, n_nbr=8)
+    t2 = time.time()
+    fit = ls.fit(train, pq_label=256, k=k, epochs=8, batch=1024, lr=2e-3, verbose=False)
+    log(f"learned predictor trained in {time.time() - t2:.1f}s: top1 acc {fit['train_top1_acc']:.3f} "
+        f"mean conf {fit['mean_conf']:.3f} mean settle {fit['mean_settle']:.1f}")
+    fcal, fcap = None, None
+    if args.fallback_quantile > 0:
+        fcal = ls.calibrate_fallback_cap(train, pq, k=k, quantile=args.fallback_quantile)
+        fcap = fcal["cap"]
+        log(f"fallback cap {fcap} expansions (settle quantile {args.fallback_quantile} over {fcal['queries']} "
+            f"training queries, settle mean {fcal['settle_mean']}, max {fcal['settle_max']})")
+
+    def ltiers(lpq):
+        # the learned queries read BF16 rows (the form's storage); the fallback tier is the plain
+        # search and reads its FP32 rows
+        if lpq >= pq:
+            return None
+        return [song_amd.tier(lpq), song_amd.tier(pq, storage=song_amd.STORE_F32)]
+
+    def make(n, lpq, scale, tiers, cap):
+        return learned.LearnedBatch(ls, n, lpq, pq, k=k, conf_threshold=0.9, step_scale=scale,
+                                    storage=song_amd.STORE_BF16, mode=song_amd.MODE_FP32, fallback_cap=cap,
+                                    tiers=tiers)
+
+    # sweep on the selection split, fallback uncapped (the headline form)
+    lst = torch.empty((ns, 8), dtype=torch.int32, device=dev)
+    lsweep, cands = [], []
+    for lpq in [int(x) for x in args.learned_pqs.split(",")]:
+        for sc in [float(x) for x in args.step_scales.split(",")]:
+            scale = sc if sc > 0 else 1e6
+            lb = make(ns, lpq, scale, ltiers(lpq), None)
+            plan = lb.run(qs, stats=lst)
+            torch.cuda.synchronize()
+            rl = recall(lb.ids, gs_d, selq)
+            ms = timed(lambda: lb.run(qs), 3, torch)
+            stl = lst.cpu().numpy()
+            fb = float((~plan[3]).float().mean().item())
+            ent = {"pq": lpq, "step_scale": sc, "recall@10_selection": round(rl, 4),
+                   "qps_selection": round(ns / (ms / 1e3), 1), "fallback_frac": round(fb, 4),
+                   "mean_expansions": round(float(stl[:, 0].mean()), 1),
+                   "mean_distances": round(float(stl[:, 1].mean()), 1)}
+            lsweep.append(ent)
+            log(f"learned+bf16 pq={lpq} scale={sc} selection recall={rl:.4f} {ms:.2f} ms "
+                f"({ent['qps_selection']:.0f} q/s) fallback={fb:.3f} exp={ent['mean_expansions']}")
+            if rl >= args.target_recall:
+                cands.append((ent, lpq, scale))
+            lb.batch.close()
+    if not cands:
+        return {"error": "no learned setting reached the target recall on the selection split", "sweep": lsweep}
+    ent, lpq, scale = max(cands, key=lambda c: c[0]["qps_selection"])
+
+    # autotune the tiers on the selection queries: the fallback tier must keep its results (it is
+    # the plain search); the confident tier may take a Bloom filter if the selection recall stays
+    # at the target
+    tt = time.time()
+    lb0 = make(ns, lpq, scale, ltiers(lpq), None)
+    p = lb0.run(qs)
+    torch.cuda.synchronize()
+    base_ids = lb0.ids.clone()
+    lb0.batch.close()
+    gt_sel = gs_d
+
+    def judge(tier_ids, rows):
+        trial = base_ids.clone()
+        trial[torch.from_numpy(rows.astype(np.int64)).to(dev)] = tier_ids[:, :k]
+        return recall(trial, gt_sel, selq) >= args.target_recall
+
+    approx = {0: judge} if ltiers(lpq) is not None and not args.no_bloom else None
+    table = autotune.tune(ix, qs, p[0], torch.full((ns,), k, dtype=torch.int32, device=dev), tiers=ltiers(lpq),
+                          mode=song_amd.MODE_FP32, storage=song_amd.STORE_BF16, reps=3, max_queries_per_tier=ns,
+                          k_ld=k, entry=p[1], steps=p[2], freeze=("storage",), approx_ok=approx)
+    log(f"learned tiers autotuned on the selection split in {time.time() - tt:.1f}s: {table['tiers']}")
+
+    def measure_test(cap):
+        lb = make(nq, lpq, scale, table["tiers"], cap)
+        st = torch.empty((nq, 8), dtype=torch.int32, device=dev)
+        pe = lb.run(q, stats=st)
+        torch.cuda.synchronize()
+        eager_ids = lb.ids.clone()
+        stl = st.cpu().numpy()
+        plan_g = lb.capture(q)
+        lb.replay()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(lb.ids, eager_ids))
+        fbm = ~plan_g[3]
+        n_fb = int(fbm.sum().item())
+        fb_capped = int(((lb.flags[fbm] & song_amd.FLAG_STEP_CAP) != 0).sum().item())
+        fb_equal = int((lb.ids[fbm][:, :k] == plain_ids[fbm][:, :k]).all(1).sum().item())
+        r_final = recall(lb.ids, gt_d, query)
+        ms_l, wall_l = timed_steps(lb.replay, args.steps, args.warmup, torch, stream)
+        # algorithmic bytes: confident queries read BF16 rows + scale, fallback queries FP32 rows
+        conf = plan_g[3].cpu().numpy()
+        alg = (alg_bytes(stl[conf], dim, dim * 2 + 4, int(conf.sum())) +
+               alg_bytes(stl[~conf], dim, dim * 4, int((~conf).sum())))
+        lb.batch.close()
+        return {"qps": round(nq / (ms_l / 1e3), 1), "ms_per_step": round(ms_l, 4), "wall_ms_per_step": round(wall_l, 4),
+                "recall@10": round(r_final, 4), "graph_ids_equal_eager": same, "algorithmic_bytes_per_step": alg,
+                "fallback": {"queries": n_fb, "cap": cap, "capped": fb_capped, "top10_ids_equal_plain": fb_equal}}
+
+    head = measure_test(None)
+    log(f"learned+bf16 batched pq={lpq} scale={ent['step_scale']} (uncapped fallback): {head['qps']:.0f} q/s "
+        f"({head['qps'] / qps_plain:.2f}x plain), test recall {head['recall@10']:.4f}")
+    capped = None
+    if fcap:
+        capped = measure_test(fcap)
+        capped["calibration"] = fcal
+        log(f"same setting, fallback capped at {fcap}: {capped['qps']:.0f} q/s, test recall {capped['recall@10']:.4f}, "
+            f"{capped['fallback']['capped']} capped, {capped['fallback']['top10_ids_equal_plain']} of "
+            f"{capped['fallback']['queries']} with the plain top-10")
+    fit_out = {kk: vv for kk, vv in fit.items() if kk != "loss"}
+    fit_out["final_loss"] = fit["loss"][-1]
+    info = dict(head, pq=lpq, step_scale=ent["step_scale"], chosen_on="selection split", selection=ent,
+                tiers=table["tiers"], capped_fallback=capped, training=fit_out, sweep_selection=lsweep)
+    return info
 
 
 def bench_c4(args, torch, song_amd):
@@ -584,10 +680,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--pqs", default="256,384,512,640,768,896,1024")
+    ap.add_argument("--pq-step", type=int, default=16,
+                    help="plain form: granularity of the bisection after the coarse grid")
+    ap.add_argument("--sel-queries", type=int, default=10_000,
+                    help="held-out selection queries (same mixture, own stream) on which settings are chosen")
     ap.add_argument("--pq", type=int, default=0, help="plain form: skip the sweep and time this pq")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -596,11 +696,10 @@ def main():
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--learned-pqs", default="64,96,128,192,256")
     ap.add_argument("--step-scales", default="1.5,2.5,0")
-    ap.add_argument("--learned-finalists", type=int, default=3,
-                    help="learned settings (fastest in the sweep at the target recall) measured in full")
+    ap.add_argument("--no-bloom", action="store_true", help="learned form: exact visited tables only")
     ap.add_argument("--fallback-quantile", type=float, default=0.999,
-                    help="fallback expansion cap = this quantile of the plain search's top-k settle step over "
-                         "low-confidence training queries (<= 0: uncapped fallback)")
+                    help="capped variant beside the headline: fallback expansion cap = this quantile of the plain "
+                         "search's top-k settle step over low-confidence training queries (<= 0: not measured)")
     ap.add_argument("--no-recall", action="store_true",
                     help="with --pq: skip the ground truth and recall (profiling runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -609,6 +708,8 @@ def main():
     ap.add_argument("--c4-n", type=int, default=2_000_000)
     ap.add_argument("--c4-nq", type=int, default=100_000)
     ap.add_argument("--c4-recall-sample", type=int, default=10_000)
+    ap.add_argument("--c5-n", type=int, default=10_000_000)
+    ap.add_argument("--c5-nq", type=int, default=100_000)
     ap.add_argument("--c3-n", type=int, default=1_000_000)
     ap.add_argument("--c3-nq", type=int, default=1000)
     ap.add_argument("--c3-pq", type=int, default=200)
@@ -625,7 +726,7 @@ def main():
 
     import song_amd
 
-    out = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args, torch, song_amd)
+    out = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c2}[args.workload](args, torch, song_amd)
     print(json.dumps(out))
 
 

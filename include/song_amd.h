@@ -79,6 +79,12 @@ typedef struct {
     int rerank;    /* low-precision storage: exact FP32 re-rank of the pq list (default 1) */
     int visited_bits; /* visited-table slot width: 16 (compact), 32, or 0 = compact whenever the ids
                          fit (default); results do not depend on it */
+    int bloom_k;   /* visited set: 0 = the exact table (default); 1..9 = a blocked Bloom filter in its
+                      place (the reference's BlockedBloomFilter, blocked_bloomfilter.h:9-82) with
+                      bloom_k bits per id in one 32-bit word and 2^hash_log2 words (hash_log2 = 0:
+                      sized by the reference's auto_tune_bloom closed form for pq); -1 = size and
+                      bits both from that closed form.  Approximate: a false positive skips an
+                      unvisited vertex, so ids may differ from the exact search (recall-judged) */
 } song_search_opts;
 
 /* Fill opts with the defaults above. */
@@ -185,6 +191,17 @@ typedef struct {
                             ids fit; results do not depend on it */
     int storage;         /* rows this tier reads: SONG_STORE_*, or -1 = the batch's cfg.storage.  A
                             query re-run after an overflow is searched with the retry tier's storage */
+    int bloom_k;         /* visited set as song_search_opts.bloom_k (hash_log2 = the filter's words);
+                            the re-run of overflowed queries always uses the exact table */
+    int queries_per_wave; /* 1 (0): one query per 64-lane wave; 2 or 4: sub-groups of 32 or 16 lanes,
+                             each searching its own query (the reference's N_MULTIQUERY,
+                             warp_astar_accelerator.h:27-30) with its queue and best list in LDS;
+                             same results; exact visited table and graph rows of 32 slots only;
+                             shape at most 3.  The re-run uses one query per wave */
+    int multiprobe;      /* with queries_per_wave 2 or 4: expansions per step, 1 (0) or 2 (the next queue
+                            head too, its neighbours evaluated in the same gather: the reference's
+                            N_MULTIPROBE, warp_astar_accelerator.h:184-197).  2 changes the expansion
+                            order, so results are approximate (recall-judged) */
 } song_tier;
 
 typedef struct {
@@ -236,6 +253,10 @@ int song_batch_run_graph(song_batch *batch, const float *d_queries, uint64_t nq,
 
 /* Queries per tier and re-run count of the last run (n_tiers + 1 values; synchronizes). */
 int song_batch_counts(song_batch *batch, uint32_t *out, void *stream);
+
+/* The Bloom filter size of the reference's auto_tune_bloom (auto_tune_bloom.cc:65-86) for a
+ * budget pq: 2^hash_log2 32-bit words and bloom_k bits per id (song_search_opts.bloom_k = -1). */
+int song_bloom_size(int pq, int *hash_log2, int *bloom_k);
 
 /* Persistent grid chosen for tier t (t = n_tiers: the retry), in workgroups. */
 int song_batch_grid(const song_batch *batch, int t, int *groups, int *waves_per_group, int *lds_per_wave);

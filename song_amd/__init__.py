@@ -36,13 +36,14 @@ class QueryParams(ctypes.Structure):
 class SearchOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slack", ctypes.c_int), ("hash_log2", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("retry", ctypes.c_int), ("storage", ctypes.c_int),
-                ("rerank", ctypes.c_int), ("visited_bits", ctypes.c_int)]
+                ("rerank", ctypes.c_int), ("visited_bits", ctypes.c_int), ("bloom_k", ctypes.c_int)]
 
 
 class Tier(ctypes.Structure):
     _fields_ = [("pq_max", ctypes.c_int), ("hash_log2", ctypes.c_int), ("slack", ctypes.c_int), ("shape", ctypes.c_int),
                 ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int), ("visited_bits", ctypes.c_int),
-                ("storage", ctypes.c_int)]
+                ("storage", ctypes.c_int), ("bloom_k", ctypes.c_int), ("queries_per_wave", ctypes.c_int),
+                ("multiprobe", ctypes.c_int)]
 
 
 MAX_TIERS = 8
@@ -92,6 +93,7 @@ def lib():
         L.song_batch_run_graph.argtypes = run_args
         L.song_batch_counts.argtypes = [vp, vp, vp]
         L.song_batch_grid.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.song_bloom_size.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -102,6 +104,7 @@ EXPORTED_SYMBOLS = (
     "song_index_add_storage", "song_index_get_storage", "song_search_device_ex",
     "song_batch_config_default", "song_resolve_mode", "song_batch_create", "song_batch_destroy", "song_batch_run",
     "song_batch_run_graph", "song_batch_counts", "song_batch_grid",
+    "song_bloom_size",
 )
 FLAG_BAD_PARAMS = 8
 
@@ -155,12 +158,20 @@ def metric_code(metric):
 
 
 def make_opts(mode=MODE_AUTO, slack=128, hash_log2=0, max_steps=0, retry=1, storage=STORE_F32, rerank=1,
-              visited_bits=0):
+              visited_bits=0, bloom_k=0):
     o = SearchOpts()
     lib().song_search_opts_default(ctypes.byref(o))
     o.mode, o.slack, o.hash_log2, o.max_steps, o.retry = mode, slack, hash_log2, max_steps, retry
-    o.storage, o.rerank, o.visited_bits = storage, rerank, visited_bits
+    o.storage, o.rerank, o.visited_bits, o.bloom_k = storage, rerank, visited_bits, bloom_k
     return o
+
+
+def bloom_size(pq):
+    """(hash_log2, bloom_k) of the reference's auto_tune_bloom sizing for budget pq: 2^hash_log2
+    32-bit words, bloom_k bits per id."""
+    h, k = ctypes.c_int(), ctypes.c_int()
+    _check(lib().song_bloom_size(int(pq), ctypes.byref(h), ctypes.byref(k)))
+    return h.value, k.value
 
 
 class Index:
@@ -205,7 +216,7 @@ class Index:
         return codes, scales
 
     def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=128, hash_log2=0, retry=1, storage=STORE_F32,
-               rerank=1, visited_bits=0):
+               rerank=1, visited_bits=0, bloom_k=0):
         """Host buffers in and out: ids int64 [nq, k] (-1 padding), dists float32, flags uint32."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
@@ -213,7 +224,7 @@ class Index:
         ids = np.empty((nq, k), dtype=np.uint64)
         ds = np.empty((nq, k), dtype=np.float32)
         fl = np.empty(nq, dtype=np.uint32)
-        o = make_opts(mode, slack, hash_log2, 0, retry, storage, rerank, visited_bits)
+        o = make_opts(mode, slack, hash_log2, 0, retry, storage, rerank, visited_bits, bloom_k)
         _check(lib().song_search(self._h, _ptr(q), nq, pq, k, _ptr(ids), _ptr(ds), _ptr(fl), ctypes.byref(o)))
         out = ids.astype(np.int64)
         out[ids == np.uint64(0xFFFFFFFFFFFFFFFF)] = -1
@@ -221,7 +232,7 @@ class Index:
 
     def search_device(self, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
                       mode=MODE_AUTO, slack=128, hash_log2=0, retry=1, stream=None, storage=STORE_F32, rerank=1,
-                      max_steps=0, entry=None, steps=None, subset=None, visited_bits=0):
+                      max_steps=0, entry=None, steps=None, subset=None, visited_bits=0, bloom_k=0):
         """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k].
 
         entry: int32 [nq] or [nq, E] start vertices (learned form; default vertex 0);
@@ -248,7 +259,7 @@ class Index:
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
-        o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank, visited_bits)
+        o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank, visited_bits, bloom_k)
         qp = QueryParams()
         qp.pq, qp.k = (pq_arr.data_ptr() if pq_arr is not None else None), (k_arr.data_ptr() if k_arr is not None else None)
         qp.entry = entry.data_ptr() if entry is not None else None
@@ -285,14 +296,19 @@ def resolve_mode(index, queries, stream=None):
     return m.value
 
 
-TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu", "visited_bits", "storage")
+TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu", "visited_bits", "storage",
+             "bloom_k", "queries_per_wave", "multiprobe")
 
 
-def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0, visited_bits=0, storage=-1):
+def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0, visited_bits=0, storage=-1,
+         bloom_k=0, queries_per_wave=1, multiprobe=1):
     """One tier of a batch configuration (see song_tier in include/song_amd.h); storage -1 = the
-    batch's row storage."""
+    batch's row storage; bloom_k != 0: a Bloom filter in place of the exact visited table;
+    queries_per_wave 2 / 4: sub-groups of 32 / 16 lanes each searching its own query; multiprobe 2
+    (with queries_per_wave > 1): two expansions per step (approximate)."""
     return dict(pq_max=pq_max, hash_log2=hash_log2, slack=slack, shape=shape, waves_per_group=waves_per_group,
-                groups_per_cu=groups_per_cu, visited_bits=visited_bits, storage=storage)
+                groups_per_cu=groups_per_cu, visited_bits=visited_bits, storage=storage, bloom_k=bloom_k,
+                queries_per_wave=queries_per_wave, multiprobe=multiprobe)
 
 
 class Batch:
@@ -393,8 +409,9 @@ class BuildStats(ctypes.Structure):
     _fields_ = [("searches", ctypes.c_uint64), ("redone", ctypes.c_uint64), ("seconds", ctypes.c_double)]
 
 
-HOST_EXPORTED_SYMBOLS = ("song_graph_build", "song_libsvm_parse", "song_file_write", "song_file_read",
-                         "song_host_last_error")
+HOST_EXPORTED_SYMBOLS = ("song_graph_build", "song_libsvm_parse", "song_libsvm_parse_mt", "song_libsvm_write",
+                         "song_query_read", "song_file_write", "song_file_read", "song_host_last_error")
+QUERY_LIBSVM, QUERY_F32, QUERY_FVECS, QUERY_FBIN = 0, 1, 2, 3
 
 
 def host_lib():
@@ -407,6 +424,9 @@ def host_lib():
         L.song_host_last_error.restype = ctypes.c_char_p
         L.song_graph_build.argtypes = [vp, u64, i32, i32, i32, vp, ctypes.POINTER(BuildStats)]
         L.song_libsvm_parse.argtypes = [ctypes.c_char_p, i32, u64, i32, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.song_libsvm_parse_mt.argtypes = [ctypes.c_char_p, i32, u64, i32, i32, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.song_libsvm_write.argtypes = [ctypes.c_char_p, vp, u64, i32, i32]
+        L.song_query_read.argtypes = [ctypes.c_char_p, i32, i32, u64, vp, ctypes.POINTER(ctypes.c_uint64)]
         L.song_file_write.argtypes = [ctypes.c_char_p, vp, u64]
         L.song_file_read.argtypes = [ctypes.c_char_p, vp, u64]
         _host = L
@@ -430,9 +450,29 @@ def build_graph(data, metric, threads=0, return_stats=False):
     return g
 
 
-def parse_libsvm(path, rows, dim, one_based=True):
+def parse_libsvm(path, rows, dim, one_based=True, threads=0):
+    """LIBSVM text -> (float32 [rows, dim], records read); threads <= 0: the library default."""
     out = np.empty((rows, dim), dtype=np.float32)
     got = ctypes.c_uint64()
-    _hcheck(host_lib().song_libsvm_parse(path.encode(), 1 if one_based else 0, rows, dim, _ptr(out),
-                                         ctypes.byref(got)))
+    _hcheck(host_lib().song_libsvm_parse_mt(path.encode(), 1 if one_based else 0, rows, dim, threads, _ptr(out),
+                                            ctypes.byref(got)))
     return out, int(got.value)
+
+
+def write_libsvm(path, data, threads=0):
+    """Rows as one-based LIBSVM text that parse_libsvm reads back exactly (native writer)."""
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    _hcheck(host_lib().song_libsvm_write(path.encode(), _ptr(d), d.shape[0], d.shape[1], threads))
+
+
+def read_queries(path, dim, fmt=None):
+    """Query file -> float32 [rows, dim]; fmt QUERY_* or None = by extension (.f32, .fvecs, .fbin,
+    else LIBSVM)."""
+    if fmt is None:
+        fmt = (QUERY_F32 if path.endswith(".f32") else QUERY_FVECS if path.endswith(".fvecs")
+               else QUERY_FBIN if path.endswith(".fbin") else QUERY_LIBSVM)
+    n = ctypes.c_uint64()
+    _hcheck(host_lib().song_query_read(path.encode(), fmt, dim, 0, None, ctypes.byref(n)))
+    out = np.zeros((max(1, n.value), dim), dtype=np.float32)
+    _hcheck(host_lib().song_query_read(path.encode(), fmt, dim, out.shape[0], _ptr(out), ctypes.byref(n)))
+    return out[: n.value]

@@ -14,6 +14,10 @@ coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
   storage          with low-precision batch storage: the tier's rows (-1 = the batch's, or FP32 rows,
                    whose decode-free distances shorten a latency-bound tier); taken only where the
                    results stay identical, e.g. BF16 rows of integer data
+  queries_per_wave 1, or 2 / 4 queries per 64-lane wave on sub-groups of 32 / 16 lanes (same results)
+  bloom_k          (recall-judged tiers only, see approx_ok) a blocked Bloom filter in place of the
+                   exact visited table: less LDS per wave, approximate visited set
+  multiprobe       (recall-judged tiers with several queries per wave) two expansions per step
 
 Each candidate runs the tier's queries through song_batch_run and is timed with events
 (median of `reps`); a candidate is accepted only if every result equals the default
@@ -57,9 +61,16 @@ def _neighbours(value, choices):
 
 
 def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STORE_F32, rerank=1, reps=3,
-         max_queries_per_tier=20000, k_ld=None, entry=None, steps=None, verbose=False, freeze=()):
+         max_queries_per_tier=20000, k_ld=None, entry=None, steps=None, verbose=False, freeze=(), approx_ok=None):
     """Tune every tier on its own queries; returns the table (dict).  freeze: knob names kept at
-    the given tiers' values (e.g. "storage", where the rows are part of what is measured)."""
+    the given tiers' values (e.g. "storage", where the rows are part of what is measured).
+
+    approx_ok: {tier index: judge} for tiers whose results are judged by recall rather than
+    equality.  Those tiers also try the Bloom filter in place of the exact visited table
+    (bloom_k -1: the reference's auto_tune_bloom sizing, and its size knob hash_log2); a
+    candidate that changes the results is accepted only if judge(ids, query_rows) is true, where
+    ids are the tier's results (int32 cuda [n, k_ld]) and query_rows the tier's rows in
+    `queries`; knobs that keep the results are accepted, as elsewhere, only on equal results."""
     import torch
 
     t_start = time.time()
@@ -84,7 +95,9 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         dists = torch.empty((nq, k_ld), dtype=torch.float32, device=q.device)
         flags = torch.empty(nq, dtype=torch.int32, device=q.device)
 
-        def measure(cfg, want=None):
+        judge = (approx_ok or {}).get(ti)
+
+        def measure(cfg, want=None, approx=False):
             try:
                 b = Batch(index, [cfg], nq, mode=mode, storage=storage, rerank=rerank)
             except SongError as e:  # e.g. LDS beyond the limit
@@ -95,7 +108,7 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
             b.close()
             if want is not None:
                 same = all(torch.equal(a, w) for a, w in zip(out, want))
-                if not same:
+                if not same and not (approx and judge(out[0], sel)):
                     return None, "results differ"
             return ms, out
 
@@ -107,7 +120,8 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         # two rounds of coordinate descent: a knob's best value can depend on another's (a larger
         # visited table only pays off with fewer waves per group, which the first round may have
         # rejected at the smaller table)
-        knobs = [("waves_per_group", (1, 2, 4)),
+        knobs = [("queries_per_wave", (1, 2, 4)),
+                 ("waves_per_group", (1, 2, 4)),
                  ("hash_log2", tuple(range(8, 15))),
                  ("shape", SHAPES),
                  ("slack", (0, 8, 32, 128)),
@@ -115,24 +129,56 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                  ("visited_bits", (0, 32))]
         if storage != STORE_F32:
             knobs.append(("storage", (-1, STORE_F32)))
+        if judge is not None:
+            knobs.append(("bloom_k", (0, -1)))
+            knobs.append(("multiprobe", (1, 2)))
         knobs = [kc for kc in knobs if kc[0] not in freeze]
         best.setdefault("storage", -1)
+        best.setdefault("bloom_k", 0)
+        best.setdefault("queries_per_wave", 1)
+        best.setdefault("multiprobe", 1)
         for _round in range(2):
             for key, choices in knobs:
                 cur = best[key]
-                if key == "hash_log2" and cur == 0:
+                bloom = best.get("bloom_k", 0) != 0
+                if key == "hash_log2" and bloom:
+                    # the filter's words: around the reference sizing for the tier's pq
+                    from . import bloom_size
+
+                    h0 = bloom_size(t["pq_max"])[0]
+                    choices = (0,) + tuple(h for h in (h0 - 1, h0 + 1, h0 + 2) if 1 <= h <= 15)
+                elif key == "hash_log2" and cur == 0:
                     cur = -1
+                if key == "visited_bits" and bloom:
+                    continue
+                # sub-groups use the exact table; multi-probe needs sub-groups
+                if (key == "queries_per_wave" and bloom) or (key == "bloom_k" and best["queries_per_wave"] > 1):
+                    continue
+                if key == "multiprobe" and best["queries_per_wave"] == 1:
+                    continue
                 for v in _neighbours(cur, choices):
                     cand = dict(best)
                     cand[key] = v
-                    ms, _ = measure(cand, want=ref)
+                    if key == "bloom_k":
+                        cand["hash_log2"] = 0  # the exact table's or the filter's default size
+                    if key == "queries_per_wave" and v > 1:
+                        cand["shape"] = min(cand["shape"], 3) if cand["shape"] >= 0 else -1
+                    if key == "queries_per_wave" and v == 1:
+                        cand["multiprobe"] = 1
+                    if key == "shape" and cand["queries_per_wave"] > 1 and v > 3:
+                        continue
+                    approx = (key in ("bloom_k", "multiprobe") or (key == "hash_log2" and bloom)
+                              or (key == "queries_per_wave" and best["multiprobe"] > 1))
+                    ms, out = measure(cand, want=ref, approx=approx)
                     if ms is None and key == "hash_log2" and cand["waves_per_group"] > 1:
                         # a table too large for the group's LDS: try it with one wave per group
                         cand["waves_per_group"] = 1
-                        ms, _ = measure(cand, want=ref)
+                        ms, out = measure(cand, want=ref, approx=approx)
                     trials.append((dict(cand), ms))
                     if ms is not None and ms < best_ms:
                         best, best_ms = cand, ms
+                        if approx:
+                            ref = out  # later knobs must keep the accepted filter's results
         tiers[ti] = best
         entry_log = {"tier": ti, "pq_max": t["pq_max"], "queries": int(nq), "ms": best_ms,
                      "qps": nq / (best_ms * 1e-3), "config": best,

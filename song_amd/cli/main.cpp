@@ -2,13 +2,14 @@
 //   song <build/test> <build_data> <query_data> <search_top_k> <row> <dim> <return_top_k> <l2/ip/cos>
 // build: parse the one-based LIBSVM rows, build the graph (byte-identical to the reference
 //        insertion build), write bfsg.data and bfsg.graph to the working directory.
-// test:  load bfsg.data / bfsg.graph, search every query of the LIBSVM file in one GPU
-//        batch, print the first <return_top_k> ids of each result, "%zu " each, one line
-//        per query (main.cu:28-33).
+// test:  load bfsg.data / bfsg.graph, search every query of the LIBSVM file (or a binary
+//        .f32 / .fvecs / .fbin query file) in one GPU batch, print the first <return_top_k>
+//        ids of each result, "%zu " each, one line per query (main.cu:28-33).
 // The search budget pq is SONG_PQ when fill_parameters.sh compiled one in (the reference
 // GPU binary's TOPK), otherwise <search_top_k> (the reference CPU binary's k).
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <string>
 
@@ -22,6 +23,18 @@ using namespace song_cli;
 static void usage(char **argv) {
     printf("Usage: %s <build/test> <build_data> <query_data> <search_top_k> <row> <dim> <return_top_k> <l2/ip/cos>\n",
            argv[0]);
+}
+
+static bool ends_with(const std::string &s, const char *suf) {
+    const size_t n = strlen(suf);
+    return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
+}
+
+static int query_format(const std::string &path) {
+    if (ends_with(path, ".f32")) return SONG_QUERY_F32;
+    if (ends_with(path, ".fvecs")) return SONG_QUERY_FVECS;
+    if (ends_with(path, ".fbin")) return SONG_QUERY_FBIN;
+    return SONG_QUERY_LIBSVM;
 }
 
 template <int M>
@@ -43,22 +56,15 @@ static int run(const std::string &mode, char **argv, size_t row, int dim, int pq
     fprintf(stderr, "...");
     g.load();
     fprintf(stderr, "done\n");
-    // the query file: count lines, parse into dense rows, search as one batch
-    FILE *fp = fopen(argv[3], "r");
-    if (!fp) {
-        fprintf(stderr, "File not found at (%s)\n", argv[3]);
-        return 1;
-    }
-    // records as fgets reads them (parser.h:57): every '\n', plus a last line without one
-    size_t lines = 0;
-    int last = '\n';
-    for (int c; (c = fgetc(fp)) != EOF; last = c)
-        if (c == '\n') ++lines;
-    if (last != '\n') ++lines;
-    fclose(fp);
-    std::vector<float> dense(std::max<size_t>(lines, 1) * (size_t)dim);
+    // the query file: LIBSVM text (the reference's format), or binary rows by extension
+    // (.f32 raw float32 rows, .fvecs, .fbin); count the records, read them into dense rows,
+    // search as one batch
+    const int fmt = query_format(argv[3]);
+    uint64_t lines = 0;
+    check_host(song_query_read(argv[3], fmt, dim, 0, nullptr, &lines), "count queries");
+    std::vector<float> dense(std::max<uint64_t>(lines, 1) * (size_t)dim);
     uint64_t nq = 0;
-    check_host(song_libsvm_parse(argv[3], 1, std::max<size_t>(lines, 1), dim, dense.data(), &nq), "parse queries");
+    check_host(song_query_read(argv[3], fmt, dim, std::max<uint64_t>(lines, 1), dense.data(), &nq), "read queries");
     std::vector<std::vector<std::pair<int, value_t>>> queries(nq);
     for (uint64_t i = 0; i < nq; ++i)
         for (int j = 0; j < dim; ++j)

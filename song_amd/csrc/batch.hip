@@ -63,6 +63,8 @@ struct Launch {
     Plan plan;
     SearchFn fn = nullptr;
     int groups = 0, waves = 1;
+    int qpw = 1;  // queries per wave
+    int multiprobe = 1;
     size_t wave_lds = 0;
     uint32_t *gvis = nullptr;
     int ghlog = 13;  // spill table per wave: 2^ghlog u32
@@ -129,6 +131,9 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     t.groups_per_cu = 0;
     t.visited_bits = 0;
     t.storage = -1;
+    t.bloom_k = 0;
+    t.queries_per_wave = 1;
+    t.multiprobe = 1;
     c->tier[0] = t;
     c->retry = t;
     c->retry.hash_log2 = 0;  // the largest tier's filter x 4
@@ -153,8 +158,11 @@ extern "C" int song_resolve_mode(const song_index *ix, const float *d_queries, u
 
 static int check_tier(const song_tier &t, int idx) {
     if (t.pq_max < 1 || t.pq_max > 4096) return fail(SONG_E_ARG, "tier %d: pq_max=%d not in [1,4096]", idx, t.pq_max);
-    if (t.hash_log2 != 0 && (t.hash_log2 < 8 || t.hash_log2 > 16))
+    if (t.bloom_k < -1 || t.bloom_k > 9) return fail(SONG_E_ARG, "tier %d: bloom_k=%d not in [-1,9]", idx, t.bloom_k);
+    if (t.bloom_k == 0 && t.hash_log2 != 0 && (t.hash_log2 < 8 || t.hash_log2 > 16))
         return fail(SONG_E_ARG, "tier %d: hash_log2=%d not in [8,16]", idx, t.hash_log2);
+    if (t.bloom_k != 0 && (t.hash_log2 < 0 || t.hash_log2 > 15))
+        return fail(SONG_E_ARG, "tier %d: Bloom filter of 2^%d words not in [2^1,2^15]", idx, t.hash_log2);
     if (t.slack < 0 || t.slack > 4096) return fail(SONG_E_ARG, "tier %d: slack=%d", idx, t.slack);
     if (t.shape < -1 || t.shape > 4) return fail(SONG_E_ARG, "tier %d: shape=%d not in [-1,4]", idx, t.shape);
     if (t.waves_per_group != 1 && t.waves_per_group != 2 && t.waves_per_group != 4)
@@ -163,6 +171,11 @@ static int check_tier(const song_tier &t, int idx) {
         return fail(SONG_E_ARG, "tier %d: groups_per_cu=%d", idx, t.groups_per_cu);
     if (t.visited_bits != 0 && t.visited_bits != 16 && t.visited_bits != 32)
         return fail(SONG_E_ARG, "tier %d: visited_bits=%d not 0, 16 or 32", idx, t.visited_bits);
+    if (t.queries_per_wave != 0 && t.queries_per_wave != 1 && t.queries_per_wave != 2 && t.queries_per_wave != 4)
+        return fail(SONG_E_ARG, "tier %d: queries_per_wave=%d not 1, 2 or 4", idx, t.queries_per_wave);
+    if (t.multiprobe < 0 || t.multiprobe > 2) return fail(SONG_E_ARG, "tier %d: multiprobe=%d not 1 or 2", idx, t.multiprobe);
+    if (t.multiprobe == 2 && t.queries_per_wave < 2)
+        return fail(SONG_E_ARG, "tier %d: multi-probe runs with several queries per wave", idx);
     if (t.storage < -1 || t.storage > SONG_STORE_FP8)
         return fail(SONG_E_ARG, "tier %d: storage=%d not -1 or a SONG_STORE_* value", idx, t.storage);
     return SONG_OK;
@@ -198,6 +211,7 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
     song_tier rt = c.retry;
     const int pq_top = c.tier[c.n_tiers - 1].pq_max;
     rt.pq_max = pq_top;
+    rt.bloom_k = 0;  // re-runs use the exact table
     {
         int rc = check_tier(rt, c.n_tiers);
         if (rc != SONG_OK) return rc;
@@ -218,28 +232,38 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         return code;
     };
     const bool exact = c.mode == SONG_MODE_EXACT;
+    // the re-run's default table is 4x the largest exact tier's (or the default for the top pq
+    // when every tier uses a Bloom filter)
     int top_hlog = 0;
+    for (int t = 0; t < c.n_tiers; ++t)
+        if (c.tier[t].bloom_k == 0)
+            top_hlog = std::max(top_hlog, c.tier[t].hash_log2 ? c.tier[t].hash_log2 : default_tier_hlog(c.tier[t].pq_max));
+    if (top_hlog == 0) top_hlog = default_tier_hlog(pq_top);
     for (int t = 0; t <= c.n_tiers; ++t) {
         const bool is_retry = t == c.n_tiers;
         const song_tier &ti = is_retry ? rt : c.tier[t];
-        int hlog = ti.hash_log2 ? ti.hash_log2 : default_tier_hlog(ti.pq_max);
+        int hlog = ti.hash_log2 ? ti.hash_log2 : (ti.bloom_k ? 0 : default_tier_hlog(ti.pq_max));
         if (is_retry && !ti.hash_log2) hlog = std::min(16, top_hlog + 2);
-        if (!is_retry) top_hlog = std::max(top_hlog, hlog);
         Launch L;
         L.pq_max = ti.pq_max;
         const int store = ti.storage >= 0 ? ti.storage : c.storage;
-        int rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
+        const int qpw = is_retry ? 1 : std::max(1, ti.queries_per_wave);
+        int rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits, ti.bloom_k,
+                           qpw);
         // the re-run's default table (the largest tier's, 4x) may not fit one wave's LDS: take the
         // largest that does (the re-run also has a 4x larger global spill table)
         while (rc == SONG_E_OVERFLOW && is_retry && !ti.hash_log2 && hlog > top_hlog) {
             --hlog;
-            rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits);
+            rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits, 0);
         }
         if (rc != SONG_OK) return cleanup(rc);
         L.fn = search_fn(ix->metric, L.plan, true);
         L.waves = ti.waves_per_group;
+        L.qpw = 64 / L.plan.lanes;
+        L.multiprobe = is_retry ? 1 : std::max(1, ti.multiprobe);
+        // one wave's LDS (lanes 64), or one query's slice (several queries per wave)
         L.wave_lds = (L.plan.lds + 15) & ~(size_t)15;
-        const size_t lds = L.wave_lds * (size_t)L.waves;
+        const size_t lds = L.wave_lds * (size_t)L.waves * (size_t)L.qpw;
         if (lds > ix->max_lds)
             return cleanup(fail(SONG_E_OVERFLOW, "tier %d: %d waves x %zu bytes of LDS exceed %zu", t, L.waves,
                                 L.wave_lds, ix->max_lds));
@@ -255,16 +279,19 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
             per_cu = nb;
         }
         L.groups = per_cu * b->n_cu;
-        const size_t slots = (size_t)L.groups * L.waves;
+        const size_t slots = (size_t)L.groups * L.waves * L.qpw;
         // the re-run gets spill tables 4x larger than the tiers'
         L.ghlog = is_retry ? std::min(18, spill + 2) : spill;
         if (hipMalloc(&L.gvis, slots * ((size_t)4 << L.ghlog)) != hipSuccess)
             return cleanup(fail(SONG_E_HIP, "tier %d: spill tables (%zu slots)", t, slots));
         b->allocs.push_back(L.gvis);
-        L.bag_stride = song::bag_stride(L.plan);
-        if (hipMalloc(&L.gbag, slots * L.bag_stride) != hipSuccess)
-            return cleanup(fail(SONG_E_HIP, "tier %d: queue bags (%zu slots)", t, slots));
-        b->allocs.push_back(L.gbag);
+        // queue bags: the one-query-per-wave kernel's (the sub-group kernel keeps its queue in LDS)
+        L.bag_stride = L.qpw > 1 ? 0 : song::bag_stride(L.plan);
+        if (L.qpw == 1) {
+            if (hipMalloc(&L.gbag, slots * L.bag_stride) != hipSuccess)
+                return cleanup(fail(SONG_E_HIP, "tier %d: queue bags (%zu slots)", t, slots));
+            b->allocs.push_back(L.gbag);
+        }
         b->tiers.push_back(L);
     }
     const int T = c.n_tiers;
@@ -382,6 +409,8 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
         p.gbag_stride = L.bag_stride;
         p.wave_lds = (uint32_t)L.wave_lds;
         p.vis16 = L.plan.vis16;
+        p.bloom_k = L.plan.bloom_k;
+        p.multiprobe = L.multiprobe;
         p.idbits = ix->idbits;
         // with tiers running side by side, the largest-pq tier (launched first, the longest
         // searches) gets issue priority
@@ -399,7 +428,7 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
             p.retry_list = nullptr;
             p.retry_count = nullptr;
         }
-        hipLaunchKernelGGL(L.fn, dim3((unsigned)L.groups), dim3(64 * L.waves), L.wave_lds * L.waves, ls, p);
+        hipLaunchKernelGGL(L.fn, dim3((unsigned)L.groups), dim3(64 * L.waves), L.wave_lds * L.waves * L.qpw, ls, p);
         HIP_TRY(hipGetLastError());
         if (ls != st) HIP_TRY(hipEventRecord(b->ev[o], ls));
     }
@@ -487,6 +516,6 @@ extern "C" int song_batch_grid(const song_batch *b, int t, int *groups, int *wav
     const Launch &L = b->tiers[t];
     if (groups) *groups = L.groups;
     if (waves_per_group) *waves_per_group = L.waves;
-    if (lds_per_wave) *lds_per_wave = (int)L.wave_lds;
+    if (lds_per_wave) *lds_per_wave = (int)(L.wave_lds * L.qpw);
     return SONG_OK;
 }

@@ -2,3 +2,6 @@
 #pragma once
 
 int host_fail(int code, const char *fmt, ...);
+
+// worker threads: `threads` if positive, else SONG_THREADS, else min(16, cores)
+int host_threads(int threads);

@@ -57,17 +57,27 @@ struct Plan {
     bool exact;
     int store, shape, hlog, slack, qcap, qld;
     int vis16;   // compact visited table (16-bit slots)
-    size_t lds;  // LDS bytes of one wave
+    int bloom_k; // > 0: blocked Bloom filter (2^hlog 32-bit words, bloom_k bits per id) instead
+    int lanes;   // lanes per query: 64 (one query per wave, search_query) or 16 / 32 (search_group)
+    size_t lds;  // LDS bytes of one wave (lanes 64) or of one query's slice (lanes 16 / 32)
 };
 
-size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16);
+size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k = 0);
 // bytes of one wave slot's queue bag (qcap keys, 256-byte aligned)
 size_t bag_stride(const Plan &p);
 // shape < 0: by row width; otherwise the given lanes-per-row shape
 // visited_bits: 16 or 32 = slot width (16 only where the ids fit, else 32); 0 = 16 where they fit
+// bloom_k: 0 = exact visited table; > 0 = blocked Bloom filter with that many bits per id and
+// 2^hlog words; -1 = Bloom filter with bits (and, for hlog 0, size) from bloom_size
+// queries_per_wave: 1 (search_query), 2 or 4 (search_group with 32 / 16 lanes per query; exact
+// visited table only, persistent form only)
 int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape = -1,
-              int visited_bits = 0);
+              int visited_bits = 0, int bloom_k = 0, int queries_per_wave = 1);
+// LDS bytes of one query's slice in search_group (its layout: visited | queue | best | stage | cid | qv)
+size_t group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, int vis16, int lanes);
 int default_hlog(int pq);
+// Bloom filter sizing of the reference's auto_tune_bloom (auto_tune_bloom.cc:6-37, 65-86)
+void bloom_size(int pq, int *hlog32, int *k);
 SearchFn search_fn(int metric, const Plan &p, bool persistent = false);
 // integer test and range of a device float array (synchronizes st)
 int device_range(const float *d, uint64_t count, int *integer, float *vmin, float *vmax, hipStream_t st);

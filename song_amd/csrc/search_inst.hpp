@@ -49,5 +49,9 @@ inline SearchFn pick_kernel(bool exact, int store, int shape, bool persistent) {
 SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent);
 SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent);
 SearchFn search_fn_cos(bool exact, int store, int shape, bool persistent);
+// defined in group_{l2,ip,cos}.hip: lanes = 16 or 32 lanes per query
+SearchFn group_fn_l2(bool exact, int store, int shape, int lanes);
+SearchFn group_fn_ip(bool exact, int store, int shape, int lanes);
+SearchFn group_fn_cos(bool exact, int store, int shape, int lanes);
 
 }  // namespace song

@@ -19,7 +19,8 @@
 //
 // Per-wave LDS layout (bytes, in order):
 //   visited  : H slots, no deletion: compact 16-bit entries in groups of 8 (vis16_insert) or
-//              32-bit (id + 1) in groups of 4 (vis_insert), one 16-byte read per group
+//              32-bit (id + 1) in groups of 4 (vis_insert), one 16-byte read per group; or,
+//              approximate, a blocked Bloom filter of 2^hlog 32-bit words (bloom_insert)
 //   scratch  : 64 keys    merge staging of the queue head (the head itself is one key per
 //                          lane in registers, ascending over lanes [h, h + nq), nq <= 64)
 //   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
@@ -97,6 +98,10 @@ struct SearchParams {
     uint32_t *retry_list;
     uint32_t *retry_count;
     int wave_prio;          // persistent kernel: 1 = raise the waves' issue priority (s_setprio 2)
+    int bloom_k;            // 0: exact visited table; 1..9: blocked Bloom filter of 2^hlog 32-bit words,
+                            // bloom_k bits per id in one word (approximate: false positives skip vertices)
+    int multiprobe;         // several-queries-per-wave kernel: expansions per step (1, or 2: the next
+                            // queue head too; approximate)
 };
 
 // ---------------------------------------------------------------- keys
@@ -325,6 +330,38 @@ __device__ __forceinline__ bool gvis_insert(uint32_t *g, uint32_t mask, int hlog
         if (old == key) return false;
         s = (s + 1u) & mask;
     }
+}
+
+// Blocked Bloom filter (the reference's BlockedBloomFilter, blocked_bloomfilter.h:9-82, with
+// one 32-bit word per block): an id sets k bits of one word, all chosen by two murmur3
+// finalizer mixes of the id -- the word from the top wlog bits of the first, the bit positions
+// from 5-bit slices of the second (positions 7..9 from the first's low bits, below the word
+// index for wlog <= 17).  Insert = one LDS atomic OR; the id counts as new unless all k bits
+// were already set.  Lanes inserting the same id see exactly one "new" (the ORs serialise).
+// No overflow and no deletion; a false positive skips a vertex that was never visited.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+__device__ __forceinline__ uint32_t bloom_mask(uint32_t h1, uint32_t h2, int k) {
+    uint32_t m = 0u;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        if (i < k) m |= 1u << ((h2 >> (5 * i)) & 31u);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (6 + i < k) m |= 1u << ((h1 >> (5 * i)) & 31u);
+    return m;
+}
+__device__ __forceinline__ bool bloom_insert(uint32_t *bf, int wlog, int k, uint32_t id) {
+    const uint32_t h1 = fmix32(id * 2u + 1u), h2 = fmix32(id ^ 0x9E3779B9u);
+    const uint32_t mask = bloom_mask(h1, h2, k);
+    const uint32_t old = atomicOr(&bf[h1 >> (32 - wlog)], mask);
+    return (old & mask) != mask;
 }
 
 // ---------------------------------------------------------------- distances
@@ -742,7 +779,8 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     const uint32_t vis_limit = (H >> 2) * 3u;
 
     uint32_t *vis = reinterpret_cast<uint32_t *>(smem);
-    const uint32_t vis_bytes = H * (P.vis16 ? 2u : 4u);
+    const bool bloom = P.bloom_k > 0;
+    const uint32_t vis_bytes = bloom ? (4u << P.hlog) : H * (P.vis16 ? 2u : 4u);
     const int gbits = P.hlog - 3;                               // compact table: groups of 8
     Key *S = reinterpret_cast<Key *>(smem + vis_bytes);        // merge scratch, QH keys
     Key *B = S + QH;
@@ -802,7 +840,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             sid = 0u;
         }
         int ins = 0;
-        if (use) ins = P.vis16 ? vis16_insert(vis, gbits, P.idbits, sid) : (vis_insert(vis, hmask, P.hlog, sid) ? 1 : 0);
+        if (use)
+            ins = bloom ? (bloom_insert(vis, P.hlog, P.bloom_k, sid) ? 1 : 0)
+                        : P.vis16 ? vis16_insert(vis, gbits, P.idbits, sid) : (vis_insert(vis, hmask, P.hlog, sid) ? 1 : 0);
         // no room for a start vertex (up to 64 in an empty table): the query is flagged and not
         // searched; the epilogue reports it for a re-run with a larger table
         if (ballot(ins == 2) != 0ull) flags |= F_VIS_OVF;
@@ -931,7 +971,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         // neighbours of `now` in row order (graph.h:196-205)
         const int deg = (int)rl32(r, 0);
         adj_words += l0 * (1u + (uint32_t)deg);
-        if (!spill && nvis + 64u > vis_limit) {
+        if (!bloom && !spill && nvis + 64u > vis_limit) {
             if (!P.gvis) {
                 flags |= F_VIS_OVF;
                 return true;
@@ -944,7 +984,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         }
         bool fresh = false, vovf = false;
         if (lane >= 1 && lane <= deg) {
-            if (!spill) {
+            if (bloom) {
+                fresh = bloom_insert(vis, P.hlog, P.bloom_k, r);
+            } else if (!spill) {
                 if (P.vis16) {
                     const int ins = vis16_insert(vis, gbits, P.idbits, r);
                     fresh = ins == 1;
@@ -957,7 +999,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                 fresh = !in_lds && gvis_insert(gv, gmask, P.ghlog, r);
             }
         }
-        if (P.vis16 && ballot(vovf) != 0ull) {
+        if (!bloom && P.vis16 && ballot(vovf) != 0ull) {
             // the compact table had no room near these ids: continue in the global table
             if (!P.gvis) {
                 flags |= F_VIS_OVF;

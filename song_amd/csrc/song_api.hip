@@ -85,6 +85,7 @@ extern "C" void song_search_opts_default(song_search_opts *o) {
     o->storage = SONG_STORE_F32;
     o->rerank = 1;
     o->visited_bits = 0;
+    o->bloom_k = 0;
 }
 
 // Low-precision copy of one row per thread: BF16 with a power-of-two scale (round to
@@ -308,18 +309,68 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 // must match the per-wave layout of search_query: visited (2^hlog slots of 4 or, compact, 2
 // bytes) | head merge scratch (64 keys) | best (pq rounded to even) | 256 candidate ids / radix
 // bins | query (the head is in registers, the bag in global memory)
-size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16) {
+size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k) {
     (void)qcap;
     size_t key = exact ? 16 : 8;
-    return ((size_t)(vis16 ? 2 : 4) << hlog) + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 +
-           (size_t)qld * 4;
+    const size_t vis = bloom_k > 0 ? ((size_t)4 << hlog) : ((size_t)(vis16 ? 2 : 4) << hlog);
+    return vis + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4;
+}
+
+// auto_tune_bloom.cc:65-86 restated: n = 12.897 pq expected insertions; the smallest filter of
+// m = 64 * 2^i bits (i = 1..15) whose false-positive rate p = (1 - e^(-k/r))^k, r = m / n,
+// k = round(ln 2 * r) (auto_tune_bloom.cc:6-37) is below 0.03, else i = 15.  Returned as
+// 2^(i + 1) 32-bit words and k (at most 9, the kernel's limit).
+void song::bloom_size(int pq, int *hlog32, int *k) {
+    const double n = (double)pq * 12.897;
+    int bi = 15;
+    double bk = 0;
+    for (int i = 1; i <= 15; ++i) {
+        const double m = 64.0 * (double)(1 << i);
+        const double r = m / n;
+        const double kk = std::round(std::log(2.0) * r);
+        const double p = std::pow(1.0 - std::exp(-kk / r), kk);
+        bi = i;
+        bk = kk;
+        if (p < 0.03) break;
+    }
+    *hlog32 = bi + 1;
+    *k = (int)std::max(1.0, std::min(9.0, bk));
+}
+
+extern "C" int song_bloom_size(int pq, int *hash_log2, int *bloom_k) {
+    if (pq < 1 || !hash_log2 || !bloom_k) return fail(SONG_E_ARG, "pq=%d", pq);
+    bloom_size(pq, hash_log2, bloom_k);
+    return SONG_OK;
 }
 
 size_t song::bag_stride(const Plan &p) { return ((size_t)p.qcap * (p.exact ? 16 : 8) + 255) & ~(size_t)255; }
 
+size_t song::group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, int vis16, int lanes) {
+    const size_t key = exact ? 16 : 8;
+    const size_t qc = (size_t)pq + (size_t)slack + 2 * (size_t)lanes;
+    return ((size_t)(vis16 ? 2 : 4) << hlog) + qc * key + (size_t)((pq + 1) & ~1) * key + 64 * key + 64 * 4 +
+           (size_t)qld * 4;
+}
+
 int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape,
-                    int visited_bits) {
+                    int visited_bits, int bloom_k, int queries_per_wave) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
+    if (queries_per_wave != 1 && queries_per_wave != 2 && queries_per_wave != 4)
+        return fail(SONG_E_ARG, "queries_per_wave=%d not 1, 2 or 4", queries_per_wave);
+    if (queries_per_wave > 1 && bloom_k != 0)
+        return fail(SONG_E_ARG, "several queries per wave use the exact visited table (bloom_k must be 0)");
+    if (queries_per_wave > 1 && ix->gstride != 32)
+        return fail(SONG_E_ARG, "several queries per wave need graph rows of 32 slots (shift <= 5)");
+    p->lanes = 64 / queries_per_wave;
+    if (bloom_k < -1 || bloom_k > 9) return fail(SONG_E_ARG, "bloom_k=%d not in [-1,9]", bloom_k);
+    if (bloom_k != 0) {
+        int bh = 0, bk = 0;
+        bloom_size(pq, &bh, &bk);
+        if (hlog <= 0) hlog = bh;
+        if (bloom_k < 0) bloom_k = bk;
+        if (hlog < 1 || hlog > 15) return fail(SONG_E_ARG, "Bloom filter of 2^%d words", hlog);
+    }
+    p->bloom_k = bloom_k;
     if (store != SONG_STORE_F32 && !ix->d_lp[store]) return fail(SONG_E_ARG, "storage %d not built", store);
     p->exact = exact && store == SONG_STORE_F32;
     p->store = store;
@@ -332,8 +383,13 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
     p->qld = (ix->dim + 15) & ~15;
     // compact visited slots when an id's bits beyond the group index (groups of 8) number 0..13
     const int lob = ix->idbits - (hlog - 3);
-    p->vis16 = (visited_bits != 32 && hlog >= 4 && lob >= 0 && lob <= 13) ? 1 : 0;
-    p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16);
+    p->vis16 = (bloom_k == 0 && visited_bits != 32 && hlog >= 4 && lob >= 0 && lob <= 13) ? 1 : 0;
+    if (p->lanes < 64) {
+        p->shape = std::min(p->shape, 3);  // a row's lanes must lie in one sub-group
+        p->lds = group_lds_bytes(p->exact, pq, slack, hlog, p->qld, p->vis16, p->lanes);
+    } else {
+        p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16, bloom_k);
+    }
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
     return SONG_OK;
@@ -347,6 +403,13 @@ int song::default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
 
 SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
+    if (p.lanes < 64) {
+        switch (metric) {
+        case SONG_L2: return group_fn_l2(p.exact, p.store, p.shape, p.lanes);
+        case SONG_IP: return group_fn_ip(p.exact, p.store, p.shape, p.lanes);
+        default: return group_fn_cos(p.exact, p.store, p.shape, p.lanes);
+        }
+    }
     switch (metric) {
     case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape, persistent);
     case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape, persistent);
@@ -369,6 +432,7 @@ static int launch(song_index *ix, const Plan &p, SearchParams sp, unsigned count
     sp.qld = p.qld;
     sp.wave_lds = (uint32_t)p.lds;
     sp.vis16 = p.vis16;
+    sp.bloom_k = p.bloom_k;
     sp.idbits = ix->idbits;
     sp.data_lp = ix->d_lp[p.store];
     sp.scale = ix->d_scale[p.store];
@@ -454,8 +518,9 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
     if (rcg != SONG_OK) return rcg;
     int hlog = opts.hash_log2 > 0 ? opts.hash_log2 : default_hlog(pq);
     hlog = std::max(8, std::min(hlog, 16));
+    if (opts.bloom_k != 0) hlog = opts.hash_log2 > 0 ? std::min(opts.hash_log2, 15) : 0;  // 0: sized for pq
     Plan plan;
-    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan, -1, opts.visited_bits);
+    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan, -1, opts.visited_bits, opts.bloom_k);
     if (rc != SONG_OK) return rc;
 
     uint32_t *flags = d_flags;
@@ -528,7 +593,8 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
             Plan p2;
             int h2 = vis ? plan.hlog + 2 : plan.hlog;
             int s2 = que ? plan.slack * 4 + 64 : plan.slack;
-            rc = make_plan(ix, exact, opts.storage, pq, s2, std::min(h2, 16), &p2, -1, opts.visited_bits);
+            rc = make_plan(ix, exact, opts.storage, pq, s2, std::min(h2, plan.bloom_k ? 15 : 16), &p2, -1,
+                           opts.visited_bits, plan.bloom_k);
             if (rc != SONG_OK) break;
             plan = p2;
             if (!d_idx && hipMalloc(&d_idx, nq * sizeof(uint32_t)) != hipSuccess) {

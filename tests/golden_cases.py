@@ -34,7 +34,8 @@ GENERATORS = {
 
 def names():
     """Cases whose vectors, graph and ids fit a quick test (the 1M case has its own tests)."""
-    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json") and not f.startswith("c2_1m"))
+    return sorted(f[:-5] for f in os.listdir(GOLDEN)
+                  if f.endswith(".json") and not f.startswith(("c2_1m", "auto_tune_bloom")))
 
 
 def load_c2_1m():

@@ -354,3 +354,69 @@ def test_ip_batch_fp32_mode_equals_plain_fp32(ip96):
     b = song_amd.Batch(c["ix"], IP_CONFIGS["tier_per_pq"], c["q"].shape[0], mode=song_amd.MODE_FP32)
     _same(_batch_run(b, c["q"], c["k_d"], c["pq_d"]), want, "ip fp32")
     b.close()
+
+
+# ---------------------------------------------------------------- several queries per wave
+
+GROUP_CONFIGS = {
+    "qpw4": [tier(p, queries_per_wave=4) for p in PQS],
+    "qpw2_two_tiers": [tier(100, queries_per_wave=2, shape=1, waves_per_group=2),
+                       tier(512, queries_per_wave=4, slack=8, hash_log2=9)],
+    "qpw4_mixed_with_one_query_tier": [tier(128, queries_per_wave=4, shape=3, groups_per_cu=1),
+                                       tier(512, waves_per_group=2)],
+}
+
+
+@pytest.mark.parametrize("name", sorted(GROUP_CONFIGS))
+def test_several_queries_per_wave_equal_plain_and_reference(c1, name):
+    """Sub-groups of 16 / 32 lanes each search their own query (exact mode, cosine 16-d): every
+    query equals the plain search at its (k, pq) -- ids, distances, flags -- through graph
+    replays, and the golden queries equal the reference's ids."""
+    print(f"[batch-group] config {name}", flush=True)
+    want = _plain(c1["ix"], c1["q"], c1["k"], c1["pq"], song_amd.MODE_EXACT)
+    b = song_amd.Batch(c1["ix"], GROUP_CONFIGS[name], c1["q"].shape[0], mode=song_amd.MODE_EXACT)
+    for graph in (False, True):
+        got = _batch_run(b, c1["q"], c1["k_d"], c1["pq_d"], graph=graph)
+        _same(got, want, f"{name} graph={graph}")
+    ng, pq = c1["ng"], c1["meta"]["pq"]
+    assert np.array_equal(got[0][:ng, :pq].cpu().numpy().astype(np.int64), c1["ref_ids"][:, :pq])
+    b.close()
+
+
+@pytest.mark.parametrize("qpw", [2, 4])
+def test_several_queries_per_wave_on_ip_and_fp32(ip96, qpw):
+    c = ip96
+    for mode in (song_amd.MODE_EXACT, song_amd.MODE_FP32):
+        want = _plain(c["ix"], c["q"], c["k"], c["pq"], mode)
+        b = song_amd.Batch(c["ix"], [tier(p, queries_per_wave=qpw) for p in (32, 64, 128, 256, 512)],
+                           c["q"].shape[0], mode=mode)
+        _same(_batch_run(b, c["q"], c["k_d"], c["pq_d"]), want, f"ip qpw={qpw} mode={mode}")
+        b.close()
+
+
+def test_several_queries_per_wave_with_entries_caps_and_spills():
+    """Learned-form inputs (several start vertices per query, expansion caps) and a visited
+    table small enough to spill, on L2 integer data in FP32 mode and BF16 rows."""
+    import torch
+
+    meta, base, query, _, graph = golden_cases.load("tie_l2_3k")
+    ix = song_amd.Index(base, graph, "l2")
+    ix.add_storage(song_amd.STORE_BF16)
+    dev = _dev()
+    q = torch.from_numpy(np.ascontiguousarray(query)).to(dev)
+    nq = q.shape[0]
+    rng = np.random.default_rng(9)
+    entry = torch.from_numpy(rng.integers(0, base.shape[0], size=(nq, 8)).astype(np.int32)).to(dev)
+    entry[::5, 3:] = -1
+    steps = torch.from_numpy(rng.integers(5, 200, size=nq).astype(np.int32)).to(dev)
+    k_h = np.full(nq, 10, dtype=np.int32)
+    pq_h = np.where(np.arange(nq) % 3 == 0, 40, 100).astype(np.int32)
+    k_d, pq_d = torch.from_numpy(k_h).to(dev), torch.from_numpy(pq_h).to(dev)
+    for storage in (song_amd.STORE_F32, song_amd.STORE_BF16):
+        want = _plain(ix, q, k_h, pq_h, song_amd.MODE_FP32, storage=storage, entry=entry, steps=steps)
+        for tiers in ([tier(40, queries_per_wave=4), tier(100, queries_per_wave=4, hash_log2=8)],
+                      [tier(100, queries_per_wave=2, hash_log2=8, slack=0)]):
+            b = song_amd.Batch(ix, tiers, nq, mode=song_amd.MODE_FP32, storage=storage)
+            got = _batch_run(b, q, k_d, pq_d, entry=entry, steps=steps)
+            _same(got, want, f"storage={storage} tiers={tiers}")
+            b.close()

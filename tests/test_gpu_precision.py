@@ -76,3 +76,26 @@ def test_precision_recall_delta_is_small(factor):
         r = groundtruth.recall_at_k(ids, gt_d, 10, base, query)
         print(f"[precision] storage={storage} recall@10={r:.4f} plain={r_plain:.4f}", flush=True)
         assert r >= r_plain - 0.05
+
+
+@pytest.mark.parametrize("storage", [song_amd.STORE_BF16, song_amd.STORE_FP8])
+@pytest.mark.parametrize("qpw", [2, 4])
+def test_several_queries_per_wave_low_precision_equals_plain(factor, storage, qpw):
+    """Sub-group search on BF16 / FP8 rows with the exact FP32 re-rank (float data, where the
+    re-rank changes distances) returns the plain search's ids and distances."""
+    import torch
+
+    base, query, graph, ix = factor
+    q = torch.from_numpy(query).cuda()
+    nq, pq, k = q.shape[0], 64, 20
+    want_i = torch.empty((nq, k), dtype=torch.int32, device=q.device)
+    want_d = torch.empty((nq, k), dtype=torch.float32, device=q.device)
+    ix.search_device(q, pq, k, want_i, want_d, storage=storage, mode=song_amd.MODE_FP32)
+    b = song_amd.Batch(ix, [song_amd.tier(pq, queries_per_wave=qpw)], nq, mode=song_amd.MODE_FP32, storage=storage)
+    ids = torch.full((nq, k), -7, dtype=torch.int32, device=q.device)
+    ds = torch.zeros((nq, k), dtype=torch.float32, device=q.device)
+    fl = torch.empty(nq, dtype=torch.int32, device=q.device)
+    b.run(q, ids, fl, pq=pq, k=k, dists=ds)
+    torch.cuda.synchronize()
+    b.close()
+    assert torch.equal(ids, want_i) and torch.equal(ds, want_d)
