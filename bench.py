@@ -34,7 +34,8 @@ random stream -- and then timed and scored on the test queries.
 inner product, 100,000 queries with uneven k and pq); `--workload c3` the precision
 form (factor-model 960-d L2: recall deltas and distance errors of BF16 / FP8 rows);
 `--workload c5` the C2 measurement on the same generator at 10,000,000 rows and
-100,000 queries.
+100,000 queries; `--workload c1` the 15k x 16 cosine workload with one and with several
+queries per wavefront.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -271,7 +272,10 @@ def bench_c2(args, torch, song_amd):
     qps_plain = nq / (ms_plain / 1e3)
     plain_info = {"pq": pq, "recall@10": round(r, 4), "qps": round(qps_plain, 1), "ms_per_step": round(ms_plain, 4),
                   "wall_ms_per_step": round(wall_plain, 4), "mean_expansions": round(float(st[:, 0].mean()), 1),
-                  "mean_distances": round(float(st[:, 1].mean()), 1), "arithmetic": "fp32 (exact on integer data)",
+                  "mean_distances": round(float(st[:, 1].mean()), 1),
+                  "max_expansions": int(st[:, 0].max()), "mean_head_refills": round(float(st[:, 5].mean()), 2),
+                  "mean_bag_truncations": round(float(st[:, 6].mean()), 2),
+                  "arithmetic": "fp32 (exact on integer data)",
                   "pq_chosen_on": "selection split" if not args.pq else "--pq"}
     log(f"plain form: {qps_plain:.0f} q/s at pq={pq}, test recall={r:.4f}")
     alg_plain = alg_bytes(st, dim, dim * 4, nq)
@@ -461,6 +465,32 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
                           mode=song_amd.MODE_FP32, storage=song_amd.STORE_BF16, reps=3, max_queries_per_tier=ns,
                           k_ld=k, entry=p[1], steps=p[2], freeze=("storage",), approx_ok=approx)
     log(f"learned tiers autotuned on the selection split in {time.time() - tt:.1f}s: {table['tiers']}")
+    # the tuner times each tier alone; in the step the tiers run side by side and the step ends with
+    # the fallback tier, which the learned tier's waves slow down.  The learned tier's grid is
+    # therefore also chosen on the whole step (results do not depend on it)
+    if ltiers(lpq) is not None:
+        def step_ms(tiers):
+            lb = make(ns, lpq, scale, tiers, None)
+            lb.run(qs)
+            ms = timed(lambda: lb.run(qs), 5, torch)
+            lb.batch.close()
+            return ms
+
+        best_t, best_ms = table["tiers"], step_ms(table["tiers"])
+        grid_log = [(table["tiers"][0]["groups_per_cu"], round(best_ms, 3))]
+        for gpc in (1, 2, 4):
+            if gpc == table["tiers"][0]["groups_per_cu"]:
+                continue
+            cand = [dict(t) for t in table["tiers"]]
+            cand[0]["groups_per_cu"] = gpc
+            ms = step_ms(cand)
+            grid_log.append((gpc, round(ms, 3)))
+            if ms < best_ms:
+                best_t, best_ms = cand, ms
+        table["tiers"] = best_t
+        table["step_grid_trials"] = grid_log
+        log(f"learned tier grid on the whole step (groups per CU, ms): {grid_log} -> "
+            f"{best_t[0]['groups_per_cu']}")
 
     def measure_test(cap):
         lb = make(nq, lpq, scale, table["tiers"], cap)
@@ -483,10 +513,13 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
         conf = plan_g[3].cpu().numpy()
         alg = (alg_bytes(stl[conf], dim, dim * 2 + 4, int(conf.sum())) +
                alg_bytes(stl[~conf], dim, dim * 4, int((~conf).sum())))
+        fb_st = stl[~conf]
         lb.batch.close()
         return {"qps": round(nq / (ms_l / 1e3), 1), "ms_per_step": round(ms_l, 4), "wall_ms_per_step": round(wall_l, 4),
                 "recall@10": round(r_final, 4), "graph_ids_equal_eager": same, "algorithmic_bytes_per_step": alg,
-                "fallback": {"queries": n_fb, "cap": cap, "capped": fb_capped, "top10_ids_equal_plain": fb_equal}}
+                "fallback": {"queries": n_fb, "cap": cap, "capped": fb_capped, "top10_ids_equal_plain": fb_equal,
+                             "max_expansions": int(fb_st[:, 0].max()) if len(fb_st) else 0,
+                             "mean_expansions": round(float(fb_st[:, 0].mean()), 1) if len(fb_st) else 0.0}}
 
     head = measure_test(None)
     log(f"learned+bf16 batched pq={lpq} scale={ent['step_scale']} (uncapped fallback): {head['qps']:.0f} q/s "
@@ -501,7 +534,8 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
     fit_out = {kk: vv for kk, vv in fit.items() if kk != "loss"}
     fit_out["final_loss"] = fit["loss"][-1]
     info = dict(head, pq=lpq, step_scale=ent["step_scale"], chosen_on="selection split", selection=ent,
-                tiers=table["tiers"], capped_fallback=capped, training=fit_out, sweep_selection=lsweep)
+                tiers=table["tiers"], step_grid_trials=table.get("step_grid_trials"), capped_fallback=capped,
+                training=fit_out, sweep_selection=lsweep)
     return info
 
 
@@ -611,6 +645,67 @@ def bench_c4(args, torch, song_amd):
     }
 
 
+def bench_c1(args, torch, song_amd):
+    """Small-row workload: 15,000 x 16 uniform rows, cosine, 5,000 queries, pq 100, top-10, exact
+    mode (ids equal the reference CPU search).  Measures what several queries per wavefront buy on
+    16-d rows (four 16-byte units, so one query leaves most of a 64-lane wave idle): the batched
+    search autotuned with one query per wave, then with 2 or 4 queries per wave allowed, against the
+    plain one-launch search."""
+    from song_amd import autotune, datasets, groundtruth, workloads
+
+    dev = torch.device("cuda:0")
+    base, query, graph, metric, _ = workloads.load("c1_15k")
+    ix = song_amd.Index(base, graph, metric)
+    nq, pq, k = query.shape[0], args.c1_pq, args.k
+    q = torch.from_numpy(query).to(dev)
+    mode = song_amd.MODE_EXACT
+    ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    fl = torch.empty(nq, dtype=torch.int32, device=dev)
+    ix.search_device(q, pq, k, ids, None, fl, mode=mode)
+    torch.cuda.synchronize()
+    plain_ids = ids.clone()
+    ms_plain, _ = timed_steps(lambda: ix.search_device(q, pq, k, ids, None, fl, mode=mode, retry=0), args.steps,
+                              args.warmup, torch)
+    _, gt_d = groundtruth.knn_fp64(base, query, k, metric)
+    r = groundtruth.recall_at_k(plain_ids.cpu().numpy().astype(np.int64), gt_d, k, base, query, metric)
+    pq_d = torch.full((nq,), pq, dtype=torch.int32, device=dev)
+    k_d = torch.full((nq,), k, dtype=torch.int32, device=dev)
+    res = {}
+    side = torch.cuda.Stream()
+    for name, freeze in (("one_query_per_wave", ("queries_per_wave",)), ("several_queries_per_wave", ())):
+        table = autotune.tune(ix, q, pq_d, k_d, mode=mode, reps=3, max_queries_per_tier=nq, k_ld=k, freeze=freeze)
+        b = song_amd.Batch(ix, table["tiers"], nq, mode=mode)
+        bids = torch.empty_like(ids)
+        bfl = torch.empty_like(fl)
+        side.wait_stream(torch.cuda.current_stream())
+        b.run(q, bids, bfl, pq=pq, k=k, graph=True, stream=side.cuda_stream)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(bids, plain_ids))
+        ms, _ = timed_steps(lambda: b.run(q, bids, bfl, pq=pq, k=k, graph=True, stream=side.cuda_stream), args.steps,
+                            args.warmup, torch, side)
+        res[name] = {"qps": round(nq / (ms / 1e3), 1), "ms_per_step": round(ms, 4), "ids_equal_plain": same,
+                     "tiers": table["tiers"], "grid": b.grid(0)}
+        log(f"c1 batched, {name}: {res[name]['qps']:.0f} q/s, ids equal plain: {same}, {table['tiers']}")
+        b.close()
+    cpu = None
+    if not args.no_cpu:
+        pids = plain_ids.cpu().numpy().astype(np.int64)
+        cpu = cpu_baseline(base, graph, metric, query, [(pq, np.arange(nq))], args.cpu_seconds,
+                           check=lambda p, take, cids: bool(np.array_equal(cids[:, :k].astype(np.int64), pids[take])))
+    best = res["several_queries_per_wave"]
+    return {
+        "metric": "QPS, 15k x16 cosine, pq 100, top-10, exact ids, 1 GPU",
+        "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 products, fp64 sums (exact)", "data": "synthetic: seeded uniform rows, graph = reference insertion build",
+        "config": {"workload": f"15,000 x 16 uniform [-1,1] cosine, {nq:,} queries, pq {pq}, top-{k}",
+                   "plain": {"qps": round(nq / (ms_plain / 1e3), 1), "ms_per_step": round(ms_plain, 4),
+                             "recall@10": round(r, 4)},
+                   "batched": res},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+
+
 def bench_c3(args, torch, song_amd):
     """Precision form on its workload: rank-64 Gaussian factor model + N(0, 0.05) noise, min-max
     scaled to [0,1], dim 960, L2, 1,000 queries.  Rows searched as FP32 (float64 accumulation =
@@ -680,7 +775,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2")
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4", "c5"), default="c2")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--pqs", default="256,384,512,640,768,896,1024")
@@ -708,6 +803,7 @@ def main():
     ap.add_argument("--c4-n", type=int, default=2_000_000)
     ap.add_argument("--c4-nq", type=int, default=100_000)
     ap.add_argument("--c4-recall-sample", type=int, default=10_000)
+    ap.add_argument("--c1-pq", type=int, default=100)
     ap.add_argument("--c5-n", type=int, default=10_000_000)
     ap.add_argument("--c5-nq", type=int, default=100_000)
     ap.add_argument("--c3-n", type=int, default=1_000_000)
@@ -726,7 +822,8 @@ def main():
 
     import song_amd
 
-    out = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c2}[args.workload](args, torch, song_amd)
+    out = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4,
+           "c5": bench_c2}[args.workload](args, torch, song_amd)
     print(json.dumps(out))
 
 

@@ -202,6 +202,9 @@ typedef struct {
                             head too, its neighbours evaluated in the same gather: the reference's
                             N_MULTIPROBE, warp_astar_accelerator.h:184-197).  2 changes the expansion
                             order, so results are approximate (recall-judged) */
+    int bag_in_lds;      /* one query per wave, float32 rows: 1 keeps the queue's overflow bag in the
+                            wave's LDS instead of global memory (same results; for latency-bound
+                            tiers with few waves per CU, whose refills then wait on LDS, not HBM) */
 } song_tier;
 
 typedef struct {

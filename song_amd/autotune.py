@@ -18,6 +18,7 @@ coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
   bloom_k          (recall-judged tiers only, see approx_ok) a blocked Bloom filter in place of the
                    exact visited table: less LDS per wave, approximate visited set
   multiprobe       (recall-judged tiers with several queries per wave) two expansions per step
+  bag_in_lds       one query per wave on float32 rows: the queue's overflow bag in LDS (same results)
 
 Each candidate runs the tier's queries through song_batch_run and is timed with events
 (median of `reps`); a candidate is accepted only if every result equals the default
@@ -126,7 +127,8 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                  ("shape", SHAPES),
                  ("slack", (0, 8, 32, 128)),
                  ("groups_per_cu", (0, 1, 2, 4, 8)),
-                 ("visited_bits", (0, 32))]
+                 ("visited_bits", (0, 32)),
+                 ("bag_in_lds", (0, 1))]
         if storage != STORE_F32:
             knobs.append(("storage", (-1, STORE_F32)))
         if judge is not None:
@@ -137,6 +139,8 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         best.setdefault("bloom_k", 0)
         best.setdefault("queries_per_wave", 1)
         best.setdefault("multiprobe", 1)
+        best.setdefault("bag_in_lds", 0)
+        rows_f32 = storage == STORE_F32 or best.get("storage", -1) == STORE_F32
         for _round in range(2):
             for key, choices in knobs:
                 cur = best[key]
@@ -156,6 +160,8 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                     continue
                 if key == "multiprobe" and best["queries_per_wave"] == 1:
                     continue
+                if key == "bag_in_lds" and (best["queries_per_wave"] > 1 or not rows_f32):
+                    continue
                 for v in _neighbours(cur, choices):
                     cand = dict(best)
                     cand[key] = v
@@ -165,6 +171,8 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                         cand["shape"] = min(cand["shape"], 3) if cand["shape"] >= 0 else -1
                     if key == "queries_per_wave" and v == 1:
                         cand["multiprobe"] = 1
+                    if key == "queries_per_wave" and v > 1:
+                        cand["bag_in_lds"] = 0
                     if key == "shape" and cand["queries_per_wave"] > 1 and v > 3:
                         continue
                     approx = (key in ("bloom_k", "multiprobe") or (key == "hash_log2" and bloom)

@@ -134,6 +134,7 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     t.bloom_k = 0;
     t.queries_per_wave = 1;
     t.multiprobe = 1;
+    t.bag_in_lds = 0;
     c->tier[0] = t;
     c->retry = t;
     c->retry.hash_log2 = 0;  // the largest tier's filter x 4
@@ -176,6 +177,7 @@ static int check_tier(const song_tier &t, int idx) {
     if (t.multiprobe < 0 || t.multiprobe > 2) return fail(SONG_E_ARG, "tier %d: multiprobe=%d not 1 or 2", idx, t.multiprobe);
     if (t.multiprobe == 2 && t.queries_per_wave < 2)
         return fail(SONG_E_ARG, "tier %d: multi-probe runs with several queries per wave", idx);
+    if (t.bag_in_lds != 0 && t.bag_in_lds != 1) return fail(SONG_E_ARG, "tier %d: bag_in_lds=%d", idx, t.bag_in_lds);
     if (t.storage < -1 || t.storage > SONG_STORE_FP8)
         return fail(SONG_E_ARG, "tier %d: storage=%d not -1 or a SONG_STORE_* value", idx, t.storage);
     return SONG_OK;
@@ -248,8 +250,9 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         L.pq_max = ti.pq_max;
         const int store = ti.storage >= 0 ? ti.storage : c.storage;
         const int qpw = is_retry ? 1 : std::max(1, ti.queries_per_wave);
+        const int bagl = is_retry ? 0 : ti.bag_in_lds;
         int rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits, ti.bloom_k,
-                           qpw);
+                           qpw, bagl);
         // the re-run's default table (the largest tier's, 4x) may not fit one wave's LDS: take the
         // largest that does (the re-run also has a 4x larger global spill table)
         while (rc == SONG_E_OVERFLOW && is_retry && !ti.hash_log2 && hlog > top_hlog) {
@@ -286,8 +289,8 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
             return cleanup(fail(SONG_E_HIP, "tier %d: spill tables (%zu slots)", t, slots));
         b->allocs.push_back(L.gvis);
         // queue bags: the one-query-per-wave kernel's (the sub-group kernel keeps its queue in LDS)
-        L.bag_stride = L.qpw > 1 ? 0 : song::bag_stride(L.plan);
-        if (L.qpw == 1) {
+        L.bag_stride = (L.qpw > 1 || L.plan.bag_lds) ? 0 : song::bag_stride(L.plan);
+        if (L.bag_stride) {
             if (hipMalloc(&L.gbag, slots * L.bag_stride) != hipSuccess)
                 return cleanup(fail(SONG_E_HIP, "tier %d: queue bags (%zu slots)", t, slots));
             b->allocs.push_back(L.gbag);

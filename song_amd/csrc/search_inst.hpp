@@ -18,37 +18,43 @@ inline int shape_for_units(int units) {
     return 4;
 }
 
+// bagl: the persistent form with the queue's bag in LDS (float32 rows only; null otherwise)
 template <int METRIC, bool EXACT, int STORE, int SHAPE>
-inline SearchFn pick_form(bool persistent) {
+inline SearchFn pick_form(bool persistent, bool bagl) {
+    if (bagl) {
+        if constexpr (STORE == ST_F32) return persistent ? search_persistent<METRIC, EXACT, STORE, SHAPE, true> : nullptr;
+        return nullptr;
+    }
     return persistent ? search_persistent<METRIC, EXACT, STORE, SHAPE> : search_kernel<METRIC, EXACT, STORE, SHAPE>;
 }
 
 template <int METRIC, bool EXACT, int STORE>
-inline SearchFn pick_shape(int shape, bool persistent) {
+inline SearchFn pick_shape(int shape, bool persistent, bool bagl) {
     switch (shape) {
-    case 0: return pick_form<METRIC, EXACT, STORE, 0>(persistent);
-    case 1: return pick_form<METRIC, EXACT, STORE, 1>(persistent);
-    case 2: return pick_form<METRIC, EXACT, STORE, 2>(persistent);
-    case 3: return pick_form<METRIC, EXACT, STORE, 3>(persistent);
-    default: return pick_form<METRIC, EXACT, STORE, 4>(persistent);
+    case 0: return pick_form<METRIC, EXACT, STORE, 0>(persistent, bagl);
+    case 1: return pick_form<METRIC, EXACT, STORE, 1>(persistent, bagl);
+    case 2: return pick_form<METRIC, EXACT, STORE, 2>(persistent, bagl);
+    case 3: return pick_form<METRIC, EXACT, STORE, 3>(persistent, bagl);
+    default: return pick_form<METRIC, EXACT, STORE, 4>(persistent, bagl);
     }
 }
 
 template <int METRIC>
-inline SearchFn pick_kernel(bool exact, int store, int shape, bool persistent) {
-    if (exact) return pick_shape<METRIC, true, ST_F32>(shape, persistent);
+inline SearchFn pick_kernel(bool exact, int store, int shape, bool persistent, bool bagl) {
+    if (exact) return pick_shape<METRIC, true, ST_F32>(shape, persistent, bagl);
     switch (store) {
-    case ST_BF16: return pick_shape<METRIC, false, ST_BF16>(shape, persistent);
-    case ST_FP8: return pick_shape<METRIC, false, ST_FP8>(shape, persistent);
-    default: return pick_shape<METRIC, false, ST_F32>(shape, persistent);
+    case ST_BF16: return pick_shape<METRIC, false, ST_BF16>(shape, persistent, bagl);
+    case ST_FP8: return pick_shape<METRIC, false, ST_FP8>(shape, persistent, bagl);
+    default: return pick_shape<METRIC, false, ST_F32>(shape, persistent, bagl);
     }
 }
 
 // defined in search_{l2,ip,cos}.hip
 // persistent: the work-list kernel search_persistent instead of one group per query
-SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent);
-SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent);
-SearchFn search_fn_cos(bool exact, int store, int shape, bool persistent);
+// bagl: the queue's bag in LDS (persistent form, float32 rows)
+SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent, bool bagl);
+SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent, bool bagl);
+SearchFn search_fn_cos(bool exact, int store, int shape, bool persistent, bool bagl);
 // defined in group_{l2,ip,cos}.hip: lanes = 16 or 32 lanes per query
 SearchFn group_fn_l2(bool exact, int store, int shape, int lanes);
 SearchFn group_fn_ip(bool exact, int store, int shape, int lanes);

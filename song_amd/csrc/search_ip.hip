@@ -2,7 +2,7 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent) {
-    return pick_kernel<M_IP>(exact, store, shape, persistent);
+SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent, bool bagl) {
+    return pick_kernel<M_IP>(exact, store, shape, persistent, bagl);
 }
 }  // namespace song

@@ -763,7 +763,10 @@ template <> struct Shape<4> { static constexpr int SUB = 64, NV = 16; };  // wid
 // ---------------------------------------------------------------- the search kernel
 
 // One query on one wave: qi is the query, slot_q the wave's spill table, smem its LDS.
-template <int METRIC, bool EXACT, int STORE, int SHAPE>
+// BAGL: the queue's bag sits in the wave's LDS slice after the query (qcap keys) instead of in
+// global memory -- for latency-bound launches with few waves per CU, where a refill or a
+// truncation then scans LDS instead of waiting on global-memory round trips.
+template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false>
 __device__ __forceinline__ void search_query(const SearchParams &P, const int qi, const int slot_q,
                                              unsigned char *smem) {
     using Key = typename KeySel<EXACT>::T;
@@ -786,11 +789,14 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     Key *B = S + QH;
     uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));  // 256 u32
     float *qv = reinterpret_cast<float *>(cid + 256);
-    Key *Bg = reinterpret_cast<Key *>(P.gbag + (size_t)slot_q * P.gbag_stride);  // bag, qcap keys
+    // bag, qcap keys: global memory (one region per slot), or this wave's LDS after the query
+    Key *Bg = BAGL ? reinterpret_cast<Key *>(qv + P.qld)
+                   : reinterpret_cast<Key *>(P.gbag + (size_t)slot_q * P.gbag_stride);
     // the layout must fit the slice and the bag the launch allocated: otherwise flag the query
     // and touch neither (a host / kernel mismatch becomes an error, not out-of-range accesses)
-    if (reinterpret_cast<unsigned char *>(qv + P.qld) - smem > (ptrdiff_t)P.wave_lds || !P.gbag ||
-        (size_t)P.qcap * sizeof(Key) > P.gbag_stride) {
+    const unsigned char *lds_end = BAGL ? reinterpret_cast<unsigned char *>(Bg + P.qcap)
+                                        : reinterpret_cast<unsigned char *>(qv + P.qld);
+    if (lds_end - smem > (ptrdiff_t)P.wave_lds || (!BAGL && (!P.gbag || (size_t)P.qcap * sizeof(Key) > P.gbag_stride))) {
         if (lane == 0 && P.out_flags) P.out_flags[qi] = F_BAD_LAUNCH;
         return;
     }
@@ -892,7 +898,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             // refill the head with the (up to) 64 smallest bag entries
             const int c = min(QH, bn);
             refills += l0;
-            bag_fence();
+            if constexpr (BAGL) lds_fence(); else bag_fence();
             const Key sel = select64(Bg, bn);
             hq = sel;  // +inf in lanes >= c
             const Key top = sel.rl(c - 1);
@@ -901,7 +907,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             bn = keep_above(Bg, bn, top);
             T = bn > 0 ? top : Key::inf();
             pid[X] = NONE;
-            vm_drain();
+            if constexpr (!BAGL) vm_drain();
         }
         const Key now = hq.rl(h);
         // stop rule, graph.h:188: |best| == pq and best.max.d < now.d (distance only, strict)
@@ -1038,11 +1044,11 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             // (ties kept) or beyond bmax; none of them can be expanded before the stop rule
             uint64_t dcut = (nbest == pq) ? bmax : ~0ull;
             truncs += l0;
-            bag_fence();
+            if constexpr (BAGL) lds_fence(); else bag_fence();
             if (pq <= nq) dcut = min(dcut, hq.rl(h + pq - 1).dk());
             else if (pq - nq <= bn) dcut = min(dcut, select_dk(Bg, bn, pq - nq, cid));
             bn = keep_dk_upto(Bg, bn, dcut);
-            vm_drain();
+            if constexpr (!BAGL) vm_drain();
             if (bn == 0) T = Key::inf();
             if (bn + 62 > P.qcap) {  // the tie group at the pq-th distance exceeds the slack
                 flags |= F_Q_OVF;
@@ -1180,7 +1186,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
 // Persistent grid: every wave takes queries from the shared list until it is exhausted
 // (each wave reaches the exit once *work_next passes *work_count).  A workgroup holds
 // blockDim.x / 64 waves, each on its own wave_lds-byte slice of LDS and its own spill table.
-template <int METRIC, bool EXACT, int STORE, int SHAPE>
+template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false>
 __global__ __launch_bounds__(256) void search_persistent(SearchParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int w = (int)(threadIdx.x >> 6);
@@ -1199,7 +1205,7 @@ __global__ __launch_bounds__(256) void search_persistent(SearchParams P) {
         const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
         if (t >= count) break;
         const int qi = P.qidx ? (int)P.qidx[t] : (int)t;
-        search_query<METRIC, EXACT, STORE, SHAPE>(P, qi, slot, mine);
+        search_query<METRIC, EXACT, STORE, SHAPE, BAGL>(P, qi, slot, mine);
     }
 }
 

@@ -309,11 +309,11 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 // must match the per-wave layout of search_query: visited (2^hlog slots of 4 or, compact, 2
 // bytes) | head merge scratch (64 keys) | best (pq rounded to even) | 256 candidate ids / radix
 // bins | query (the head is in registers, the bag in global memory)
-size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k) {
-    (void)qcap;
+size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k, int bag_lds) {
     size_t key = exact ? 16 : 8;
     const size_t vis = bloom_k > 0 ? ((size_t)4 << hlog) : ((size_t)(vis16 ? 2 : 4) << hlog);
-    return vis + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4;
+    return vis + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4 +
+           (bag_lds ? (size_t)qcap * key : 0);
 }
 
 // auto_tune_bloom.cc:65-86 restated: n = 12.897 pq expected insertions; the smallest filter of
@@ -353,7 +353,7 @@ size_t song::group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, i
 }
 
 int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape,
-                    int visited_bits, int bloom_k, int queries_per_wave) {
+                    int visited_bits, int bloom_k, int queries_per_wave, int bag_lds) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
     if (queries_per_wave != 1 && queries_per_wave != 2 && queries_per_wave != 4)
         return fail(SONG_E_ARG, "queries_per_wave=%d not 1, 2 or 4", queries_per_wave);
@@ -362,6 +362,9 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
     if (queries_per_wave > 1 && ix->gstride != 32)
         return fail(SONG_E_ARG, "several queries per wave need graph rows of 32 slots (shift <= 5)");
     p->lanes = 64 / queries_per_wave;
+    if (bag_lds && (queries_per_wave > 1 || store != SONG_STORE_F32))
+        return fail(SONG_E_ARG, "an LDS bag needs one query per wave and float32 rows");
+    p->bag_lds = bag_lds ? 1 : 0;
     if (bloom_k < -1 || bloom_k > 9) return fail(SONG_E_ARG, "bloom_k=%d not in [-1,9]", bloom_k);
     if (bloom_k != 0) {
         int bh = 0, bk = 0;
@@ -388,7 +391,7 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
         p->shape = std::min(p->shape, 3);  // a row's lanes must lie in one sub-group
         p->lds = group_lds_bytes(p->exact, pq, slack, hlog, p->qld, p->vis16, p->lanes);
     } else {
-        p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16, bloom_k);
+        p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16, bloom_k, p->bag_lds);
     }
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
@@ -410,10 +413,11 @@ SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
         default: return group_fn_cos(p.exact, p.store, p.shape, p.lanes);
         }
     }
+    const bool bagl = p.bag_lds && persistent;
     switch (metric) {
-    case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape, persistent);
-    case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape, persistent);
-    default: return search_fn_cos(p.exact, p.store, p.shape, persistent);
+    case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape, persistent, bagl);
+    case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape, persistent, bagl);
+    default: return search_fn_cos(p.exact, p.store, p.shape, persistent, bagl);
     }
 }
 

@@ -109,6 +109,8 @@ CONFIGS = {
     # tables up to 2^15: the re-run's default (the largest tier's table x4) would not fit one
     # wave's LDS, so it takes the largest table that does
     "large_tables": [tier(128, hash_log2=14), tier(512, hash_log2=15)],
+    # the queue's overflow bag in LDS, with little tie room so that refills and truncations run
+    "lds_bag": [tier(100, bag_in_lds=1, slack=0), tier(512, bag_in_lds=1, waves_per_group=2, slack=8, shape=2)],
 }
 
 

@@ -84,3 +84,32 @@ def test_batch_tier_with_filter_equals_plain_search_with_filter(mix):
         torch.cuda.synchronize()
         assert torch.equal(ids, want), f"run {rep}"
     b.close()
+
+
+@pytest.mark.parametrize("pq", [64, 128])
+def test_multiprobe_tier_recall(mix, pq):
+    """Two expansions per step (the next queue head's neighbours evaluated in the same gather)
+    on the sub-group kernel: an approximate expansion order, so recall is what is pinned; the
+    exact sub-group tier equals the plain search."""
+    import torch
+
+    base, query, ix, gt_d = mix
+    q = torch.from_numpy(query).cuda()
+    nq, k = q.shape[0], 10
+    plain, _, _ = ix.search(query, pq, k, mode=song_amd.MODE_FP32)
+    r_e = groundtruth.recall_at_k(plain, gt_d, k, base, query)
+    out = {}
+    for mp in (1, 2):
+        b = song_amd.Batch(ix, [tier(pq, queries_per_wave=4, multiprobe=mp)], nq, mode=song_amd.MODE_FP32)
+        ids = torch.full((nq, k), -7, dtype=torch.int32, device=q.device)
+        fl = torch.empty(nq, dtype=torch.int32, device=q.device)
+        st = torch.empty((nq, 8), dtype=torch.int32, device=q.device)
+        b.run(q, ids, fl, pq=pq, k=k, stats=st)
+        torch.cuda.synchronize()
+        b.close()
+        out[mp] = (ids.cpu().numpy().astype(np.int64), st.cpu().numpy())
+    assert np.array_equal(out[1][0], plain)
+    r_m = groundtruth.recall_at_k(out[2][0], gt_d, k, base, query)
+    print(f"[multiprobe] pq {pq}: recall@10 {r_m:.4f} vs exact {r_e:.4f}; expansions "
+          f"{out[2][1][:, 0].mean():.1f} vs {out[1][1][:, 0].mean():.1f}", flush=True)
+    assert r_m >= r_e - 0.02
