@@ -265,6 +265,25 @@ int song_bloom_size(int pq, int *hash_log2, int *bloom_k);
 int song_batch_grid(const song_batch *batch, int t, int *groups, int *waves_per_group, int *lds_per_wave);
 
 /*
+ * song_search_device_ex without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  The kernels' scratch (queue bags, visited spill
+ * tables, work counter) lives in one static device array of SONG_NOALLOC_SCRATCH_BYTES on
+ * each GPU the library runs on, held until the process ends; the searches run on a
+ * persistent grid with as many wave slots as that array holds for the call's pq, slack and
+ * mode.  Differences from song_search_device_ex: opts->mode must be EXACT or FP32 (AUTO
+ * inspects the queries in allocated memory; song_resolve_mode picks beforehand),
+ * opts->retry is ignored (overflowed queries keep their flags, as with retry = 0), and a
+ * configuration whose per-slot scratch leaves fewer than one slot per CU is rejected with
+ * SONG_E_ARG.  Results, flags and stats equal song_search_device_ex's with retry = 0.
+ * Calls that overlap on one GPU race on the shared array.
+ */
+#define SONG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int song_search_device_noalloc(song_index *index, const float *d_queries, uint64_t nq, int pq, int k,
+                               const song_query_params *qp, uint32_t *d_ids, float *d_dist, int k_ld,
+                               uint32_t *d_flags, uint32_t *d_stats, const song_search_opts *opts, void *stream);
+
+/*
  * One-shot replacement of WarpAStarAccelerator::astar_multi_start_search_batch
  * (warp_astar_accelerator.h:347): dense queries (nq x dim), pq = the compiled
  * TOPK of the reference, results nq x pq uint64 ascending by (dist, id).

@@ -94,6 +94,8 @@ def lib():
         L.song_batch_counts.argtypes = [vp, vp, vp]
         L.song_batch_grid.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         L.song_bloom_size.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.song_search_device_noalloc.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(QueryParams), vp, vp, i32,
+                                                 vp, vp, ctypes.POINTER(SearchOpts), vp]
         _lib = L
     return _lib
 
@@ -103,9 +105,10 @@ EXPORTED_SYMBOLS = (
     "song_search", "song_search_device", "song_astar_multi_start_search_batch", "song_last_error",
     "song_index_add_storage", "song_index_get_storage", "song_search_device_ex",
     "song_batch_config_default", "song_resolve_mode", "song_batch_create", "song_batch_destroy", "song_batch_run",
-    "song_batch_run_graph", "song_batch_counts", "song_batch_grid",
+    "song_batch_run_graph", "song_batch_counts", "song_batch_grid", "song_search_device_noalloc",
     "song_bloom_size",
 )
+NOALLOC_SCRATCH_BYTES = 64 << 20
 FLAG_BAD_PARAMS = 8
 
 
@@ -270,6 +273,35 @@ class Index:
         _check(lib().song_search_device_ex(self._h, dp(queries), nq, pq, k, ctypes.byref(qp), dp(ids), dp(dists),
                                            ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
                                            ctypes.c_void_p(stream)))
+
+
+def search_device_noalloc(index, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
+                          mode=MODE_EXACT, slack=128, hash_log2=0, storage=STORE_F32, rerank=1, max_steps=0,
+                          entry=None, steps=None, subset=None, visited_bits=0, stream=None, bloom_k=0):
+    """song_search_device_noalloc: Index.search_device without device allocation (scratch in the
+    library's static per-GPU array); mode must be MODE_EXACT or MODE_FP32, overflowed queries keep
+    their flags (no re-run).  Argument checks as search_device; no host synchronization."""
+    import torch
+
+    nq = queries.shape[0]
+    dev = queries.device
+    _check_tensor(queries, "queries", _f32(), nq, index.dim, ndim=(2,))
+    _check_tensor(ids, "ids", _int32s(), nq, k, dev, ndim=(2,))
+    _check_tensor(dists, "dists", _f32(), nq, ids.shape[1], dev, ndim=(2,))
+    _check_tensor(flags, "flags", _int32s(), nq, None, dev, ndim=(1,))
+    _check_tensor(stats, "stats", _int32s(), nq, 8, dev, ndim=(2,))
+    for name, t in (("pq_arr", pq_arr), ("k_arr", k_arr), ("steps", steps)):
+        _check_tensor(t, name, _int32s(), nq, None, dev, ndim=(1,))
+    _check_tensor(entry, "entry", _int32s(), nq, None, dev, ndim=(1, 2))
+    _check_tensor(subset, "subset", _int32s(), 0, None, dev, ndim=(1,))
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    o = make_opts(mode, slack, hash_log2, max_steps, 0, storage, rerank, visited_bits, bloom_k)
+    qp = _query_params(pq_arr, k_arr, entry, steps, subset)
+    _check(lib().song_search_device_noalloc(index._h, dp(queries), nq, pq, k, ctypes.byref(qp), dp(ids), dp(dists),
+                                            ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
+                                            ctypes.c_void_p(stream)))
 
 
 def _query_params(pq_arr=None, k_arr=None, entry=None, steps=None, subset=None):
