@@ -230,8 +230,11 @@ def bench_c2(args, torch, song_amd):
         ms = timed(lambda: plain(qs, s_ids, s_flags, pq), 3, torch)
         sweep.append({"pq": pq, "recall@10_selection": round(r, 4), "qps_selection": round(ns / (ms / 1e3), 1)})
         log(f"plain pq={pq} selection recall@{k}={r:.4f} ({ns / (ms / 1e3):.0f} q/s)")
-        return r >= args.target_recall
+        return r >= sel_target
 
+    # settings must reach the target on the selection split with a margin, so that the test split
+    # (another sample of the same mixture) reaches it too
+    sel_target = args.target_recall + args.selection_margin
     if args.pq:
         pq = args.pq
     elif args.no_recall:
@@ -314,7 +317,7 @@ def bench_c2(args, torch, song_amd):
     learned_info = None
     if not args.no_learned and not args.no_recall:
         learned_info = bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_d, pq, k, dim,
-                                     plain_ids, qps_plain, recall, stream)
+                                     plain_ids, qps_plain, recall, stream, sel_target)
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
@@ -353,7 +356,7 @@ def bench_c2(args, torch, song_amd):
         "config": {"workload": f"{n_rows:,} x 128 L2 mixture (1,000 centres, sigma 20, clipped/rounded), "
                                f"{nq:,} queries, top-10",
                    "form": form, "k": k, "target_recall": args.target_recall,
-                   "selection_queries": ns,
+                   "selection_queries": ns, "selection_target": round(sel_target, 4),
                    "plain": plain_info,
                    "plain_batched": batched_info,
                    "learned_bf16_batched": learned_info,
@@ -380,7 +383,7 @@ def ap_error(msg):
 
 
 def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_d, pq, k, dim, plain_ids,
-                  qps_plain, recall, stream):
+                  qps_plain, recall, stream, sel_target):
     """Learned + BF16 batched form.  Trained on 200,000 queries of the mixture (stream 1); the
     learned budget and stop scale are chosen on the selection split (stream 3), the tiers are
     autotuned there (the confident tier may take a Bloom filter if the selection recall stays at
@@ -437,7 +440,7 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
             lsweep.append(ent)
             log(f"learned+bf16 pq={lpq} scale={sc} selection recall={rl:.4f} {ms:.2f} ms "
                 f"({ent['qps_selection']:.0f} q/s) fallback={fb:.3f} exp={ent['mean_expansions']}")
-            if rl >= args.target_recall:
+            if rl >= sel_target:
                 cands.append((ent, lpq, scale))
             lb.batch.close()
     if not cands:
@@ -458,7 +461,7 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
     def judge(tier_ids, rows):
         trial = base_ids.clone()
         trial[torch.from_numpy(rows.astype(np.int64)).to(dev)] = tier_ids[:, :k]
-        return recall(trial, gt_sel, selq) >= args.target_recall
+        return recall(trial, gt_sel, selq) >= sel_target
 
     approx = {0: judge} if ltiers(lpq) is not None and not args.no_bloom else None
     table = autotune.tune(ix, qs, p[0], torch.full((ns,), k, dtype=torch.int32, device=dev), tiers=ltiers(lpq),
@@ -672,8 +675,12 @@ def bench_c1(args, torch, song_amd):
     k_d = torch.full((nq,), k, dtype=torch.int32, device=dev)
     res = {}
     side = torch.cuda.Stream()
-    for name, freeze in (("one_query_per_wave", ("queries_per_wave",)), ("several_queries_per_wave", ())):
-        table = autotune.tune(ix, q, pq_d, k_d, mode=mode, reps=3, max_queries_per_tier=nq, k_ld=k, freeze=freeze)
+    runs = [("one_query_per_wave", ("queries_per_wave",), 1), ("autotuned_queries_per_wave", (), None),
+            ("two_queries_per_wave", ("queries_per_wave",), 2), ("four_queries_per_wave", ("queries_per_wave",), 4)]
+    for name, freeze, qpw in runs:
+        start = [song_amd.tier(pq, queries_per_wave=qpw)] if qpw else None
+        table = autotune.tune(ix, q, pq_d, k_d, tiers=start, mode=mode, reps=3, max_queries_per_tier=nq, k_ld=k,
+                              freeze=freeze)
         b = song_amd.Batch(ix, table["tiers"], nq, mode=mode)
         bids = torch.empty_like(ids)
         bfl = torch.empty_like(fl)
@@ -692,7 +699,7 @@ def bench_c1(args, torch, song_amd):
         pids = plain_ids.cpu().numpy().astype(np.int64)
         cpu = cpu_baseline(base, graph, metric, query, [(pq, np.arange(nq))], args.cpu_seconds,
                            check=lambda p, take, cids: bool(np.array_equal(cids[:, :k].astype(np.int64), pids[take])))
-    best = res["several_queries_per_wave"]
+    best = max(res.values(), key=lambda r: r["qps"])
     return {
         "metric": "QPS, 15k x16 cosine, pq 100, top-10, exact ids, 1 GPU",
         "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -779,6 +786,8 @@ def main():
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--pqs", default="256,384,512,640,768,896,1024")
+    ap.add_argument("--selection-margin", type=float, default=0.004,
+                    help="settings are chosen where the selection split's recall reaches the target plus this")
     ap.add_argument("--pq-step", type=int, default=16,
                     help="plain form: granularity of the bisection after the coarse grid")
     ap.add_argument("--sel-queries", type=int, default=10_000,
