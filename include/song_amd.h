@@ -205,6 +205,10 @@ typedef struct {
     int bag_in_lds;      /* one query per wave, float32 rows: 1 keeps the queue's overflow bag in the
                             wave's LDS instead of global memory (same results; for latency-bound
                             tiers with few waves per CU, whose refills then wait on LDS, not HBM) */
+    int waves_per_query; /* 1 (0), or 2: each query gets a search wave and a helper wave that computes,
+                            ahead of the search, the distances of the predicted next expansion's
+                            unvisited neighbours into an LDS cache the search wave reads (same results;
+                            FP32 mode, float32 rows, exact visited table; waves_per_group is then 1) */
 } song_tier;
 
 typedef struct {

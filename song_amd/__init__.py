@@ -43,7 +43,7 @@ class Tier(ctypes.Structure):
     _fields_ = [("pq_max", ctypes.c_int), ("hash_log2", ctypes.c_int), ("slack", ctypes.c_int), ("shape", ctypes.c_int),
                 ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int), ("visited_bits", ctypes.c_int),
                 ("storage", ctypes.c_int), ("bloom_k", ctypes.c_int), ("queries_per_wave", ctypes.c_int),
-                ("multiprobe", ctypes.c_int), ("bag_in_lds", ctypes.c_int)]
+                ("multiprobe", ctypes.c_int), ("bag_in_lds", ctypes.c_int), ("waves_per_query", ctypes.c_int)]
 
 
 MAX_TIERS = 8
@@ -329,19 +329,21 @@ def resolve_mode(index, queries, stream=None):
 
 
 TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu", "visited_bits", "storage",
-             "bloom_k", "queries_per_wave", "multiprobe", "bag_in_lds")
+             "bloom_k", "queries_per_wave", "multiprobe", "bag_in_lds", "waves_per_query")
 
 
 def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0, visited_bits=0, storage=-1,
-         bloom_k=0, queries_per_wave=1, multiprobe=1, bag_in_lds=0):
+         bloom_k=0, queries_per_wave=1, multiprobe=1, bag_in_lds=0, waves_per_query=1):
     """One tier of a batch configuration (see song_tier in include/song_amd.h); storage -1 = the
     batch's row storage; bloom_k != 0: a Bloom filter in place of the exact visited table;
     queries_per_wave 2 / 4: sub-groups of 32 / 16 lanes each searching its own query; multiprobe 2
     (with queries_per_wave > 1): two expansions per step (approximate); bag_in_lds 1 (one query
-    per wave, float32 rows): the queue's overflow bag in LDS."""
+    per wave, float32 rows): the queue's overflow bag in LDS; waves_per_query 2: a helper wave per query
+    computes the predicted next expansion's distances ahead (FP32 mode, float32 rows)."""
     return dict(pq_max=pq_max, hash_log2=hash_log2, slack=slack, shape=shape, waves_per_group=waves_per_group,
                 groups_per_cu=groups_per_cu, visited_bits=visited_bits, storage=storage, bloom_k=bloom_k,
-                queries_per_wave=queries_per_wave, multiprobe=multiprobe, bag_in_lds=bag_in_lds)
+                queries_per_wave=queries_per_wave, multiprobe=multiprobe, bag_in_lds=bag_in_lds,
+                waves_per_query=waves_per_query)
 
 
 class Batch:

@@ -19,6 +19,8 @@ coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
                    exact visited table: less LDS per wave, approximate visited set
   multiprobe       (recall-judged tiers with several queries per wave) two expansions per step
   bag_in_lds       one query per wave on float32 rows: the queue's overflow bag in LDS (same results)
+  waves_per_query  1, or 2: a helper wave per query computes the predicted next expansion's distances
+                   ahead of the search wave (FP32 mode, float32 rows; same results)
 
 Each candidate runs the tier's queries through song_batch_run and is timed with events
 (median of `reps`); a candidate is accepted only if every result equals the default
@@ -122,6 +124,7 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         # visited table only pays off with fewer waves per group, which the first round may have
         # rejected at the smaller table)
         knobs = [("queries_per_wave", (1, 2, 4)),
+                 ("waves_per_query", (1, 2)),
                  ("waves_per_group", (1, 2, 4)),
                  ("hash_log2", tuple(range(8, 15))),
                  ("shape", SHAPES),
@@ -140,6 +143,7 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         best.setdefault("queries_per_wave", 1)
         best.setdefault("multiprobe", 1)
         best.setdefault("bag_in_lds", 0)
+        best.setdefault("waves_per_query", 1)
         rows_f32 = storage == STORE_F32 or best.get("storage", -1) == STORE_F32
         for _round in range(2):
             for key, choices in knobs:
@@ -160,7 +164,13 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                     continue
                 if key == "multiprobe" and best["queries_per_wave"] == 1:
                     continue
-                if key == "bag_in_lds" and (best["queries_per_wave"] > 1 or not rows_f32):
+                if key == "bag_in_lds" and (best["queries_per_wave"] > 1 or not rows_f32 or best["waves_per_query"] > 1):
+                    continue
+                # the helper wave: FP32 mode, float32 rows, exact table, one query per wave, global bag
+                if key == "waves_per_query" and (mode == MODE_EXACT or not rows_f32 or bloom or best["queries_per_wave"] > 1
+                                                 or best["bag_in_lds"]):
+                    continue
+                if key in ("queries_per_wave", "bag_in_lds", "bloom_k") and best["waves_per_query"] > 1:
                     continue
                 for v in _neighbours(cur, choices):
                     cand = dict(best)

@@ -65,6 +65,7 @@ struct Launch {
     int groups = 0, waves = 1;
     int qpw = 1;  // queries per wave
     int multiprobe = 1;
+    int duo = 0;  // two waves per query
     size_t wave_lds = 0;
     uint32_t *gvis = nullptr;
     int ghlog = 13;  // spill table per wave: 2^ghlog u32
@@ -135,6 +136,7 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     t.queries_per_wave = 1;
     t.multiprobe = 1;
     t.bag_in_lds = 0;
+    t.waves_per_query = 1;
     c->tier[0] = t;
     c->retry = t;
     c->retry.hash_log2 = 0;  // the largest tier's filter x 4
@@ -178,6 +180,8 @@ static int check_tier(const song_tier &t, int idx) {
     if (t.multiprobe == 2 && t.queries_per_wave < 2)
         return fail(SONG_E_ARG, "tier %d: multi-probe runs with several queries per wave", idx);
     if (t.bag_in_lds != 0 && t.bag_in_lds != 1) return fail(SONG_E_ARG, "tier %d: bag_in_lds=%d", idx, t.bag_in_lds);
+    if (t.waves_per_query < 0 || t.waves_per_query > 2)
+        return fail(SONG_E_ARG, "tier %d: waves_per_query=%d not 1 or 2", idx, t.waves_per_query);
     if (t.storage < -1 || t.storage > SONG_STORE_FP8)
         return fail(SONG_E_ARG, "tier %d: storage=%d not -1 or a SONG_STORE_* value", idx, t.storage);
     return SONG_OK;
@@ -251,8 +255,9 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         const int store = ti.storage >= 0 ? ti.storage : c.storage;
         const int qpw = is_retry ? 1 : std::max(1, ti.queries_per_wave);
         const int bagl = is_retry ? 0 : ti.bag_in_lds;
+        const int wpq = is_retry ? 1 : std::max(1, ti.waves_per_query);
         int rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits, ti.bloom_k,
-                           qpw, bagl);
+                           qpw, bagl, wpq);
         // the re-run's default table (the largest tier's, 4x) may not fit one wave's LDS: take the
         // largest that does (the re-run also has a 4x larger global spill table)
         while (rc == SONG_E_OVERFLOW && is_retry && !ti.hash_log2 && hlog > top_hlog) {
@@ -261,12 +266,14 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         }
         if (rc != SONG_OK) return cleanup(rc);
         L.fn = search_fn(ix->metric, L.plan, true);
-        L.waves = ti.waves_per_group;
+        L.duo = L.plan.duo;
+        // two waves per query: one query (search wave + helper wave) per workgroup
+        L.waves = L.duo ? 1 : ti.waves_per_group;
         L.qpw = 64 / L.plan.lanes;
         L.multiprobe = is_retry ? 1 : std::max(1, ti.multiprobe);
         // one wave's LDS (lanes 64), or one query's slice (several queries per wave)
         L.wave_lds = (L.plan.lds + 15) & ~(size_t)15;
-        const size_t lds = L.wave_lds * (size_t)L.waves * (size_t)L.qpw;
+        const size_t lds = L.wave_lds * (size_t)L.waves * (size_t)L.qpw + (L.duo ? (size_t)DUO_BYTES : 0);
         if (lds > ix->max_lds)
             return cleanup(fail(SONG_E_OVERFLOW, "tier %d: %d waves x %zu bytes of LDS exceed %zu", t, L.waves,
                                 L.wave_lds, ix->max_lds));
@@ -276,7 +283,8 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         int per_cu = ti.groups_per_cu;
         if (per_cu == 0) {
             int nb = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)L.fn, 64 * L.waves, lds) != hipSuccess ||
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)L.fn, 64 * L.waves * (L.duo ? 2 : 1),
+                                                             lds) != hipSuccess ||
                 nb < 1)
                 return cleanup(fail(SONG_E_HIP, "tier %d: occupancy query failed", t));
             per_cu = nb;
@@ -431,7 +439,8 @@ static int enqueue(song_batch *b, const RunArgs &a, hipStream_t st) {
             p.retry_list = nullptr;
             p.retry_count = nullptr;
         }
-        hipLaunchKernelGGL(L.fn, dim3((unsigned)L.groups), dim3(64 * L.waves), L.wave_lds * L.waves * L.qpw, ls, p);
+        hipLaunchKernelGGL(L.fn, dim3((unsigned)L.groups), dim3(64 * L.waves * (L.duo ? 2 : 1)),
+                           L.wave_lds * L.waves * L.qpw + (L.duo ? (size_t)DUO_BYTES : 0), ls, p);
         HIP_TRY(hipGetLastError());
         if (ls != st) HIP_TRY(hipEventRecord(b->ev[o], ls));
     }

@@ -60,6 +60,7 @@ struct Plan {
     int bloom_k; // > 0: blocked Bloom filter (2^hlog 32-bit words, bloom_k bits per id) instead
     int lanes;   // lanes per query: 64 (one query per wave, search_query) or 16 / 32 (search_group)
     int bag_lds; // search_query's bag in LDS (qcap keys after the query) instead of global memory
+    int duo;     // two waves per query (search_duo: search wave + helper wave)
     size_t lds;  // LDS bytes of one wave (lanes 64) or of one query's slice (lanes 16 / 32)
 };
 
@@ -73,8 +74,11 @@ size_t bag_stride(const Plan &p);
 // queries_per_wave: 1 (search_query), 2 or 4 (search_group with 32 / 16 lanes per query; exact
 // visited table only, persistent form only)
 // bag_lds: the queue's bag in LDS (one query per wave, float32 rows, persistent launches)
+// waves_per_query: 1, or 2 (search_duo: float32 mode, float32 rows, exact visited table, bag in global
+// memory, persistent launches)
 int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape = -1,
-              int visited_bits = 0, int bloom_k = 0, int queries_per_wave = 1, int bag_lds = 0);
+              int visited_bits = 0, int bloom_k = 0, int queries_per_wave = 1, int bag_lds = 0,
+              int waves_per_query = 1);
 // LDS bytes of one query's slice in search_group (its layout: visited | queue | best | stage | cid | qv)
 size_t group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, int vis16, int lanes);
 int default_hlog(int pq);

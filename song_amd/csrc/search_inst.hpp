@@ -39,6 +39,18 @@ inline SearchFn pick_shape(int shape, bool persistent, bool bagl) {
     }
 }
 
+// two waves per query (search_duo): float32 keys on float32 rows
+template <int METRIC>
+inline SearchFn pick_duo(int shape) {
+    switch (shape) {
+    case 0: return search_duo<METRIC, ST_F32, 0>;
+    case 1: return search_duo<METRIC, ST_F32, 1>;
+    case 2: return search_duo<METRIC, ST_F32, 2>;
+    case 3: return search_duo<METRIC, ST_F32, 3>;
+    default: return search_duo<METRIC, ST_F32, 4>;
+    }
+}
+
 template <int METRIC>
 inline SearchFn pick_kernel(bool exact, int store, int shape, bool persistent, bool bagl) {
     if (exact) return pick_shape<METRIC, true, ST_F32>(shape, persistent, bagl);
@@ -55,6 +67,9 @@ inline SearchFn pick_kernel(bool exact, int store, int shape, bool persistent, b
 SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent, bool bagl);
 SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent, bool bagl);
 SearchFn search_fn_cos(bool exact, int store, int shape, bool persistent, bool bagl);
+SearchFn duo_fn_l2(int shape);
+SearchFn duo_fn_ip(int shape);
+SearchFn duo_fn_cos(int shape);
 // defined in group_{l2,ip,cos}.hip: lanes = 16 or 32 lanes per query
 SearchFn group_fn_l2(bool exact, int store, int shape, int lanes);
 SearchFn group_fn_ip(bool exact, int store, int shape, int lanes);

@@ -5,4 +5,5 @@ namespace song {
 SearchFn search_fn_ip(bool exact, int store, int shape, bool persistent, bool bagl) {
     return pick_kernel<M_IP>(exact, store, shape, persistent, bagl);
 }
+SearchFn duo_fn_ip(int shape) { return pick_duo<M_IP>(shape); }
 }  // namespace song

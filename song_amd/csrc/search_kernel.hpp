@@ -760,15 +760,43 @@ template <> struct Shape<2> { static constexpr int SUB = 4, NV = 16; };   // <= 
 template <> struct Shape<3> { static constexpr int SUB = 16, NV = 16; };  // <= 256 units
 template <> struct Shape<4> { static constexpr int SUB = 64, NV = 16; };  // wider rows, chunked
 
+// ---------------------------------------------------------------- two waves per query
+
+// Two wavefronts per query (search_duo): the search wave runs search_query; the helper wave
+// reads the search wave's predicted next expansion (the queue head it prefetches), and for that
+// vertex's neighbours that are not yet visited computes the distances into a small LDS cache
+// keyed by id, so that when the search wave expands that vertex its candidates' distances are
+// usually there and the row gather leaves its dependent chain.  The helper only reads the
+// search wave's visited table and never changes its state; a cache entry is the search key
+// itself ((distance bits << 32) | id, one 64-bit LDS store), computed by the same code as the
+// search wave's own, so results are unchanged whether or not an entry is used.
+//
+// Queries are separated by an epoch: at a new query the search wave bumps the epoch, waits
+// (bounded) for the helper's acknowledgement -- given between two of its work units, after
+// every store of the previous unit -- and only then clears and uses the cache; without the
+// acknowledgement in time it searches that query without the cache.  The helper leaves when the
+// search wave sets `done` after its last query.
+struct DuoMail {
+    uint32_t epoch, ack, tseq, tepoch, target, done, lenv_bits, pad[9];
+};
+constexpr int DUO_CACHE = 256;                 // cache entries (8 bytes each)
+constexpr int DUO_ACK_SPINS = 4096;            // the search wave's bounded wait for the acknowledgement
+constexpr int DUO_BYTES = (int)sizeof(DuoMail) + DUO_CACHE * 8 + 64 * 4;  // mailbox, cache, helper ids
+__device__ __forceinline__ uint32_t duo_slot(uint32_t id) { return (id * 0x9E3779B1u) >> 24; }
+__device__ __forceinline__ uint32_t lds_ld(const uint32_t *p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
+__device__ __forceinline__ void lds_st(uint32_t *p, uint32_t v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
 // ---------------------------------------------------------------- the search kernel
 
 // One query on one wave: qi is the query, slot_q the wave's spill table, smem its LDS.
 // BAGL: the queue's bag sits in the wave's LDS slice after the query (qcap keys) instead of in
 // global memory -- for latency-bound launches with few waves per CU, where a refill or a
 // truncation then scans LDS instead of waiting on global-memory round trips.
-template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false>
+// HELP: the search wave of search_duo (float32 keys); duo = the mailbox and cache in LDS.
+template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false, bool HELP = false>
 __device__ __forceinline__ void search_query(const SearchParams &P, const int qi, const int slot_q,
-                                             unsigned char *smem) {
+                                             unsigned char *smem, unsigned char *duo = nullptr) {
     using Key = typename KeySel<EXACT>::T;
     constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
     constexpr int QH = 64;  // head entries: one per lane, ascending from lane h
@@ -812,6 +840,40 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             for (int i = 0; i < P.dim; ++i) lenv = fmaf(qv[i], qv[i], lenv);
         lenv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(lenv)));
     }
+
+    bool use_cache = false;
+    uint32_t epoch = 0u, cache_hits = 0u;
+    DuoMail *mail = nullptr;
+    uint64_t *dcache = nullptr;
+    if constexpr (HELP) {
+        static_assert(!EXACT, "the helper's cache holds float32 keys");
+        mail = reinterpret_cast<DuoMail *>(duo);
+        dcache = reinterpret_cast<uint64_t *>(duo + sizeof(DuoMail));
+        // new query (its vector and norm are in place): bump the epoch, wait for the helper to
+        // acknowledge it, then clear the cache
+        epoch = uni(lds_ld(&mail->epoch)) + 1u;
+        if (lane == 0) {
+            lds_st(&mail->lenv_bits, __float_as_uint(lenv));
+            lds_st(&mail->epoch, epoch);
+        }
+        for (int s = 0; s < DUO_ACK_SPINS && !use_cache; ++s) {
+            use_cache = uni(lds_ld(&mail->ack)) == epoch;
+            if (!use_cache) __builtin_amdgcn_s_sleep(1);
+        }
+        if (use_cache)
+            for (int i = lane; i < DUO_CACHE; i += 64) dcache[i] = ~0ull;
+        lds_fence();
+    }
+    // the predicted next expansion, for the helper
+    const auto publish = [&](uint32_t v) {
+        if constexpr (HELP) {
+            if (use_cache && lane == 0) {
+                mail->target = v;
+                mail->tepoch = epoch;
+                lds_st(&mail->tseq, mail->tseq + 1u);
+            }
+        }
+    };
 
     uint32_t flags = 0u;
     int steps = 0;
@@ -931,6 +993,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         // leaves the prefetch in flight.
         pid[Y] = nq > 0 ? hq.rl(h).id() : NONE;
         prow[Y] = adj_row(nq > 0 ? pid[Y] : 0u);
+        if (nq > 0) publish(pid[Y]);
 
         // insert `now` into the sorted pq-best list (push + pop-max of graph.h:193-195).
         // Expansions come out in nearly ascending order, so `now` almost always lands in the
@@ -1033,7 +1096,21 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (m == 0) return false;
         lds_fence();
         const uint32_t cand = (lane < m) ? cid[lane] : 0u;
-        const Key mine = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, m, cand, cid, qv, lenv);
+        Key mine;
+        bool cached = false;
+        if constexpr (HELP) {
+            if (use_cache) {
+                // every candidate's distance in the helper's cache: no gather
+                const uint64_t ce = (lane < m) ? dcache[duo_slot(cand)] : ~0ull;
+                const bool hit = lane < m && (uint32_t)ce == cand;
+                if (ballot(lane < m && !hit) == 0ull) {
+                    mine = KeyF{(lane < m) ? ce : ~0ull};
+                    cached = true;
+                    cache_hits += l0;
+                }
+            }
+        }
+        if (!cached) mine = eval_candidates<METRIC, EXACT, STORE, SUB, NV, Key>(P, m, cand, cid, qv, lenv);
 
         // ---- the candidates: at most T into the sorted head, above T into the bag
         const uint64_t cut = (nbest == pq) ? bmax : ~0ull;  // drop distance keys above cut
@@ -1102,6 +1179,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (nq > 0 && hq.rl(h).id() != pid[Y]) {
             pid[Y] = hq.rl(h).id();
             prow[Y] = adj_row(pid[Y]);
+            publish(pid[Y]);
         }
         return false;
     };
@@ -1169,7 +1247,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             o[4] = (uint32_t)settle;
             o[5] = refills;
             o[6] = truncs;
-            o[7] = 0u;
+            o[7] = cache_hits;
         }
     }
 }
@@ -1207,6 +1285,86 @@ __global__ __launch_bounds__(256) void search_persistent(SearchParams P) {
         const int qi = P.qidx ? (int)P.qidx[t] : (int)t;
         search_query<METRIC, EXACT, STORE, SHAPE, BAGL>(P, qi, slot, mine);
     }
+}
+
+// The helper wave of search_duo (see DuoMail): slice = the search wave's LDS slice.
+template <int METRIC, int STORE, int SHAPE>
+__device__ void duo_helper(const SearchParams &P, unsigned char *slice, unsigned char *duo) {
+    constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
+    const int lane = lane_id();
+    DuoMail *mail = reinterpret_cast<DuoMail *>(duo);
+    uint64_t *dcache = reinterpret_cast<uint64_t *>(duo + sizeof(DuoMail));
+    uint32_t *hid = reinterpret_cast<uint32_t *>(dcache + DUO_CACHE);
+    // the search wave's layout (search_query): visited | 64 keys | pq keys | 256 u32 | qv
+    const uint32_t H = 1u << P.hlog;
+    const uint32_t vis_bytes = H * (P.vis16 ? 2u : 4u);
+    const uint32_t *vis = reinterpret_cast<const uint32_t *>(slice);
+    const KeyF *S = reinterpret_cast<const KeyF *>(slice + vis_bytes);
+    const uint32_t *cid = reinterpret_cast<const uint32_t *>(S + 64 + ((P.pq + 1) & ~1));
+    const float *qv = reinterpret_cast<const float *>(cid + 256);
+    uint32_t my_epoch = 0u, last_t = 0u;
+    float lenv = 0.0f;
+    for (;;) {
+        uint32_t t = 0u;
+        for (;;) {
+            if (uni(lds_ld(&mail->done))) return;
+            const uint32_t e = uni(lds_ld(&mail->epoch));
+            if (e != my_epoch) {
+                // everything this wave stored for the previous epoch is written: acknowledge
+                my_epoch = e;
+                lenv = __uint_as_float(uni(lds_ld(&mail->lenv_bits)));
+                lds_fence();
+                if (lane == 0) lds_st(&mail->ack, e);
+            }
+            t = uni(lds_ld(&mail->tseq));
+            if (t != last_t) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        last_t = t;
+        const uint32_t v = uni(mail->target), te = uni(mail->tepoch);
+        if (te != my_epoch || (uint64_t)v >= P.n) continue;
+        const uint32_t r = P.graph[(size_t)v * (size_t)P.gstride + (uint32_t)min(lane, P.gstride - 1)];
+        const int deg = (int)rl32(r, 0);
+        bool want = false;
+        if (lane >= 1 && lane <= deg)
+            want = !(P.vis16 ? vis16_test(vis, P.hlog - 3, P.idbits, r) : vis_test(vis, H - 1u, P.hlog, r));
+        const uint64_t fm = ballot(want);
+        const int m = __popcll(fm);
+        if (m == 0) continue;
+        if (want) hid[lanes_below(fm)] = r;
+        lds_fence();
+        const uint32_t cand = (lane < m) ? hid[lane] : 0u;
+        const KeyF k = eval_candidates<METRIC, false, STORE, SUB, NV, KeyF>(P, m, cand, hid, qv, lenv);
+        if (lane < m) dcache[duo_slot(cand)] = k.v;
+        lds_fence();
+    }
+}
+
+// Two waves per query, one query per workgroup at a time: wave 0 searches (the persistent work
+// loop of search_persistent), wave 1 helps (duo_helper).  LDS: the search wave's wave_lds slice,
+// then DUO_BYTES of mailbox, cache and helper ids.
+template <int METRIC, int STORE, int SHAPE>
+__global__ __launch_bounds__(128) void search_duo(SearchParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char *duo = smem + P.wave_lds;
+    if (threadIdx.x < sizeof(DuoMail) / 4) reinterpret_cast<uint32_t *>(duo)[threadIdx.x] = 0u;
+    __syncthreads();
+    if (threadIdx.x >= 64) {
+        duo_helper<METRIC, STORE, SHAPE>(P, smem, duo);
+        return;
+    }
+    const uint32_t count = *P.work_count;
+    if (P.wave_prio) __builtin_amdgcn_s_setprio(2);
+    for (;;) {
+        // every lane issues the add (see search_persistent)
+        const uint32_t v = atomicAdd(P.work_next, lane_id() == 0 ? 1u : 0u);
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+        if (t >= count) break;
+        const int qi = P.qidx ? (int)P.qidx[t] : (int)t;
+        search_query<METRIC, false, STORE, SHAPE, false, true>(P, qi, (int)blockIdx.x, smem, duo);
+    }
+    lds_fence();
+    if (lane_id() == 0) lds_st(&reinterpret_cast<DuoMail *>(duo)->done, 1u);
 }
 
 }  // namespace song

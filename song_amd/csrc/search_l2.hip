@@ -5,4 +5,5 @@ namespace song {
 SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent, bool bagl) {
     return pick_kernel<M_L2>(exact, store, shape, persistent, bagl);
 }
+SearchFn duo_fn_l2(int shape) { return pick_duo<M_L2>(shape); }
 }  // namespace song

@@ -353,7 +353,7 @@ size_t song::group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, i
 }
 
 int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape,
-                    int visited_bits, int bloom_k, int queries_per_wave, int bag_lds) {
+                    int visited_bits, int bloom_k, int queries_per_wave, int bag_lds, int waves_per_query) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
     if (queries_per_wave != 1 && queries_per_wave != 2 && queries_per_wave != 4)
         return fail(SONG_E_ARG, "queries_per_wave=%d not 1, 2 or 4", queries_per_wave);
@@ -365,6 +365,12 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
     if (bag_lds && (queries_per_wave > 1 || store != SONG_STORE_F32))
         return fail(SONG_E_ARG, "an LDS bag needs one query per wave and float32 rows");
     p->bag_lds = bag_lds ? 1 : 0;
+    if (waves_per_query != 1 && waves_per_query != 2)
+        return fail(SONG_E_ARG, "waves_per_query=%d not 1 or 2", waves_per_query);
+    if (waves_per_query == 2 && (exact || store != SONG_STORE_F32 || bloom_k != 0 || queries_per_wave > 1 || bag_lds))
+        return fail(SONG_E_ARG, "two waves per query need FP32 mode on float32 rows, the exact visited table, one "
+                                "query per wave and the bag in global memory");
+    p->duo = waves_per_query == 2 ? 1 : 0;
     if (bloom_k < -1 || bloom_k > 9) return fail(SONG_E_ARG, "bloom_k=%d not in [-1,9]", bloom_k);
     if (bloom_k != 0) {
         int bh = 0, bk = 0;
@@ -406,6 +412,13 @@ int song::default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 
 
 SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
+    if (p.duo && persistent) {
+        switch (metric) {
+        case SONG_L2: return duo_fn_l2(p.shape);
+        case SONG_IP: return duo_fn_ip(p.shape);
+        default: return duo_fn_cos(p.shape);
+        }
+    }
     if (p.lanes < 64) {
         switch (metric) {
         case SONG_L2: return group_fn_l2(p.exact, p.store, p.shape, p.lanes);

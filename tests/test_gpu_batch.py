@@ -422,3 +422,64 @@ def test_several_queries_per_wave_with_entries_caps_and_spills():
             got = _batch_run(b, q, k_d, pq_d, entry=entry, steps=steps)
             _same(got, want, f"storage={storage} tiers={tiers}")
             b.close()
+
+
+# ---------------------------------------------------------------- two waves per query
+
+def _mix20k():
+    import torch
+
+    meta, base, query, ref_ids, graph = golden_cases.load("mix_l2_20k")
+    graph = song_amd.build_graph(base, "l2", threads=8) if graph is None else graph
+    ix = song_amd.Index(base, graph, "l2")
+    return meta, base, ix, torch.from_numpy(np.ascontiguousarray(query)).to(_dev()), ref_ids
+
+
+@pytest.mark.parametrize("pq", [100, 200])
+def test_two_waves_per_query_equal_plain_and_reference(pq):
+    """A helper wave per query computes the predicted next expansion's distances ahead of the
+    search wave (FP32 mode, integer L2 data where FP32 is exact): ids, distances and flags equal
+    the plain search's, the golden queries equal the reference's ids at the golden pq, and the
+    search waves do take their distances from the helper's cache (stats column 7)."""
+    import torch
+
+    meta, base, ix, q, ref_ids = _mix20k()
+    nq = q.shape[0]
+    k = pq if pq == meta["pq"] else 20  # the full list at the golden pq
+    k_h = np.full(nq, k, dtype=np.int32)
+    pq_h = np.full(nq, pq, dtype=np.int32)
+    want = _plain(ix, q, k_h, pq_h, song_amd.MODE_FP32)
+    for tiers in ([tier(pq, waves_per_query=2)], [tier(pq, waves_per_query=2, hash_log2=9, slack=0, groups_per_cu=1)]):
+        b = song_amd.Batch(ix, tiers, nq, mode=song_amd.MODE_FP32)
+        st = torch.zeros((nq, 8), dtype=torch.int32, device=q.device)
+        for graph in (False, True):
+            ids = torch.full((nq, 100), -7, dtype=torch.int32, device=q.device)
+            ds = torch.zeros((nq, 100), dtype=torch.float32, device=q.device)
+            fl = torch.full((nq,), -1, dtype=torch.int32, device=q.device)
+            kd = torch.from_numpy(k_h).to(q.device)
+            pd = torch.from_numpy(pq_h).to(q.device)
+            if graph:
+                side = torch.cuda.Stream()
+                side.wait_stream(torch.cuda.current_stream())
+                for _ in range(2):
+                    b.run(q, ids, fl, pq_arr=pd, k_arr=kd, dists=ds, graph=True, stream=side.cuda_stream)
+            else:
+                b.run(q, ids, fl, pq_arr=pd, k_arr=kd, dists=ds, stats=st)
+            torch.cuda.synchronize()
+            _same((ids, ds, fl), want, f"duo pq={pq} {tiers} graph={graph}")
+            if pq == meta["pq"]:
+                assert np.array_equal(ids[:, :pq].cpu().numpy().astype(np.int64), ref_ids[:, :pq])
+        hits = int(st[:, 7].sum())
+        exps = int(st[:, 0].sum())
+        print(f"[duo] pq {pq}: {hits} of {exps} expansions took their distances from the helper", flush=True)
+        assert hits > 0
+        b.close()
+
+
+def test_two_waves_per_query_on_inner_product(ip96):
+    c = ip96
+    want = _plain(c["ix"], c["q"], c["k"], c["pq"], song_amd.MODE_FP32)
+    b = song_amd.Batch(c["ix"], [tier(p, waves_per_query=2) for p in (32, 64, 128, 256, 512)], c["q"].shape[0],
+                       mode=song_amd.MODE_FP32)
+    _same(_batch_run(b, c["q"], c["k_d"], c["pq_d"]), want, "ip duo")
+    b.close()
