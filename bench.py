@@ -111,8 +111,8 @@ def alg_bytes(st, dim, row_bytes, nq):
 
 def load_traffic(pq):
     """HBM bytes per launch of the plain kernel from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) when it was
-    measured at this pq; else None."""
+    summary (the kernel's L2-to-fabric read and write requests, each times its size)
+    when it was measured at this pq; else None."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -319,6 +319,44 @@ def bench_c2(args, torch, song_amd):
         learned_info = bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_d, pq, k, dim,
                                      plain_ids, qps_plain, recall, stream, sel_target)
 
+    # ---------------------------------------------------------------- the fallback tier alone
+    # the learned step ends with its fallback tier (the plain search of the low-confidence
+    # queries); here that tier's configuration runs alone on as many test queries (their
+    # expansion counts are those of typical queries), with one and with two waves per query
+    fb_alone = None
+    if args.fallback_n > 0:
+        fb_tier = None
+        n_fb = args.fallback_n
+        if learned_info and "tiers" in learned_info and len(learned_info["tiers"]) > 1:
+            fb_tier = dict(learned_info["tiers"][1])
+            n_fb = learned_info["fallback"]["queries"] or n_fb
+        if fb_tier is None:
+            fb_tier = song_amd.tier(pq, hash_log2=14, slack=128, waves_per_group=4, visited_bits=32,
+                                    storage=song_amd.STORE_F32, waves_per_query=2)
+        n_fb = min(n_fb, nq)
+        fb_alone = {"queries": n_fb, "tier": fb_tier}
+        side = torch.cuda.Stream()
+        qf = q[:n_fb].contiguous()
+        for wpq in (1, 2):
+            # one query per workgroup in both (a two-wave tier runs one search wave per group)
+            t = dict(fb_tier, waves_per_query=wpq, waves_per_group=1)
+            b = song_amd.Batch(ix, [t], n_fb, mode=mode)
+            fid = torch.empty((n_fb, k), dtype=torch.int32, device=dev)
+            ffl = torch.empty(n_fb, dtype=torch.int32, device=dev)
+            side.wait_stream(stream)
+            b.run(qf, fid, ffl, pq=pq, k=k, stream=side.cuda_stream)
+            torch.cuda.synchronize()
+            same = bool(torch.equal(fid, plain_ids[:n_fb]))
+            b.run(qf, fid, ffl, pq=pq, k=k, graph=True, stream=side.cuda_stream)
+            torch.cuda.synchronize()
+            same = same and bool(torch.equal(fid, plain_ids[:n_fb]))
+            ms_f, _ = timed_steps(lambda: b.run(qf, fid, ffl, pq=pq, k=k, graph=True, stream=side.cuda_stream),
+                                  args.steps, args.warmup, torch, side)
+            b.close()
+            fb_alone[f"waves_per_query_{wpq}"] = {"ms_per_step": round(ms_f, 4), "ids_equal_plain": same}
+            log(f"fallback tier alone ({n_fb} queries, {wpq} wave(s) per query): {ms_f:.3f} ms, ids equal plain: {same}")
+        fb_alone["max_expansions"] = int(st[:n_fb, 0].max())
+
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -361,6 +399,7 @@ def bench_c2(args, torch, song_amd):
                    "plain_batched": batched_info,
                    "learned_bf16_batched": learned_info,
                    "learned_vs_plain": round(value / qps_plain, 3) if have_learned else None,
+                   "fallback_tier_alone": fb_alone,
                    "plain_sweep_selection": sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved_plain / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK, 5),
@@ -517,12 +556,28 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
         alg = (alg_bytes(stl[conf], dim, dim * 2 + 4, int(conf.sum())) +
                alg_bytes(stl[~conf], dim, dim * 4, int((~conf).sum())))
         fb_st = stl[~conf]
+        grid0 = lb.batch.grid(0)
         lb.batch.close()
+        vf = None
+        if cap is None and table["tiers"] and table["tiers"][0].get("bloom_k", 0) != 0:
+            # the confident tier took a Bloom filter: the same setting with the exact visited table
+            ex = [dict(t) for t in table["tiers"]]
+            ex[0]["bloom_k"] = 0
+            lbx = make(nq, lpq, scale, ex, cap)
+            lbx.run(q)
+            torch.cuda.synchronize()
+            r_exact = recall(lbx.ids, gt_d, query)
+            gx = lbx.batch.grid(0)
+            lbx.batch.close()
+            vf = {"learned_tier_bloom_k": table["tiers"][0]["bloom_k"], "lds_per_wave": grid0[2],
+                  "exact_table": {"lds_per_wave": gx[2], "recall@10": round(r_exact, 4)},
+                  "recall@10_delta": round(r_final - r_exact, 4)}
         return {"qps": round(nq / (ms_l / 1e3), 1), "ms_per_step": round(ms_l, 4), "wall_ms_per_step": round(wall_l, 4),
                 "recall@10": round(r_final, 4), "graph_ids_equal_eager": same, "algorithmic_bytes_per_step": alg,
                 "fallback": {"queries": n_fb, "cap": cap, "capped": fb_capped, "top10_ids_equal_plain": fb_equal,
                              "max_expansions": int(fb_st[:, 0].max()) if len(fb_st) else 0,
-                             "mean_expansions": round(float(fb_st[:, 0].mean()), 1) if len(fb_st) else 0.0}}
+                             "mean_expansions": round(float(fb_st[:, 0].mean()), 1) if len(fb_st) else 0.0},
+                "visited_filter": vf}
 
     head = measure_test(None)
     log(f"learned+bf16 batched pq={lpq} scale={ent['step_scale']} (uncapped fallback): {head['qps']:.0f} q/s "
@@ -800,6 +855,9 @@ def main():
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--learned-pqs", default="64,96,128,192,256")
     ap.add_argument("--step-scales", default="1.5,2.5,0")
+    ap.add_argument("--fallback-n", type=int, default=442,
+                    help="C2: queries of the fallback-tier-alone measurement when the learned form does not run "
+                         "(0 = skip it)")
     ap.add_argument("--no-bloom", action="store_true", help="learned form: exact visited tables only")
     ap.add_argument("--fallback-quantile", type=float, default=0.999,
                     help="capped variant beside the headline: fallback expansion cap = this quantile of the plain "
