@@ -193,7 +193,7 @@ def bench_c2(args, torch, song_amd):
     # held-out selection split: queries from the same mixture as the test queries, on their own
     # stream (the learned form trains on stream 1); every setting is chosen on this split and then
     # timed and scored on the 10,000 test queries
-    selq = datasets.mixture_l2_queries(args.sel_queries, seed=2, stream=3)
+    selq = datasets.mixture_l2_queries(args.sel_queries or len(query), seed=2, stream=3)
     gt_d = gs_d = None
     if not args.no_recall:
         t1 = time.time()
@@ -295,7 +295,7 @@ def bench_c2(args, torch, song_amd):
         pq_s = torch.full((ns,), pq, dtype=torch.int32, device=dev)
         k_s = torch.full((ns,), k, dtype=torch.int32, device=dev)
         tb = time.time()
-        table = autotune.tune(ix, qs, pq_s, k_s, mode=mode, reps=3, max_queries_per_tier=ns, k_ld=k)
+        table = autotune.tune(ix, qs, pq_s, k_s, mode=mode, reps=3, max_queries_per_tier=min(ns, 20000), k_ld=k)
         log(f"plain batched: autotuned in {time.time() - tb:.1f}s: {table['tiers']}")
         b = song_amd.Batch(ix, table["tiers"], nq, mode=mode)
         side = torch.cuda.Stream()
@@ -379,7 +379,7 @@ def bench_c2(args, torch, song_amd):
     n_rows = base.shape[0]
     size = f"{n_rows // 1_000_000}M" if n_rows % 1_000_000 == 0 else f"{n_rows:,}"
     out = {
-        "metric": f"QPS @ recall@10>=0.95, {size} x128 L2, 1 GPU: learned/BF16 form vs plain",
+        "metric": f"QPS @ recall@10>={args.target_recall:g}, {size} x128 L2, 1 GPU: learned/BF16 form vs plain",
         "value": value,
         "unit": "queries/s",
         "n_gpus": 1,
@@ -845,8 +845,9 @@ def main():
                     help="settings are chosen where the selection split's recall reaches the target plus this")
     ap.add_argument("--pq-step", type=int, default=16,
                     help="plain form: granularity of the bisection after the coarse grid")
-    ap.add_argument("--sel-queries", type=int, default=10_000,
-                    help="held-out selection queries (same mixture, own stream) on which settings are chosen")
+    ap.add_argument("--sel-queries", type=int, default=0,
+                    help="held-out selection queries (same mixture, own stream) on which settings are chosen; "
+                         "0 = as many as test queries, so that tiers are tuned at the batch size they run at")
     ap.add_argument("--pq", type=int, default=0, help="plain form: skip the sweep and time this pq")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")

@@ -8,7 +8,7 @@ coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
   hash_log2        visited filter size in LDS (smaller: more waves fit, more spills)
   shape            lanes per candidate row (the sub-wave width of one distance)
   slack            heap depth beyond pq (tie room; overflows cost a re-run)
-  groups_per_cu    persistent grid size (0 = all that fit)
+  groups_per_cu    persistent grid size (0 = all that fit; a smaller grid must be 3% faster)
   visited_bits     visited-table slot width (0 = compact 16-bit slots where the ids fit, 32 = 32-bit:
                    fewer instructions per probe, twice the LDS per slot)
   storage          with low-precision batch storage: the tier's rows (-1 = the batch's, or FP32 rows,
@@ -193,7 +193,10 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                         cand["waves_per_group"] = 1
                         ms, out = measure(cand, want=ref, approx=approx)
                     trials.append((dict(cand), ms))
-                    if ms is not None and ms < best_ms:
+                    # a grid smaller than all that fit must win clearly: it caps the queries in
+                    # flight, which costs a batch larger than the tuning sample
+                    margin = 0.97 if key == "groups_per_cu" and v != 0 else 1.0
+                    if ms is not None and ms < best_ms * margin:
                         best, best_ms = cand, ms
                         if approx:
                             ref = out  # later knobs must keep the accepted filter's results

@@ -1349,12 +1349,13 @@ __global__ __launch_bounds__(128) void search_duo(SearchParams P) {
     unsigned char *duo = smem + P.wave_lds;
     if (threadIdx.x < sizeof(DuoMail) / 4) reinterpret_cast<uint32_t *>(duo)[threadIdx.x] = 0u;
     __syncthreads();
+    // both waves of a prioritised tier: the helper's distances are on the search wave's path
+    if (P.wave_prio) __builtin_amdgcn_s_setprio(2);
     if (threadIdx.x >= 64) {
         duo_helper<METRIC, STORE, SHAPE>(P, smem, duo);
         return;
     }
     const uint32_t count = *P.work_count;
-    if (P.wave_prio) __builtin_amdgcn_s_setprio(2);
     for (;;) {
         // every lane issues the add (see search_persistent)
         const uint32_t v = atomicAdd(P.work_next, lane_id() == 0 ? 1u : 0u);

@@ -74,13 +74,43 @@ def search_asm():
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def work_loop_is_uniform(body):
+    """As loop_head_is_uniform for a kernel with other loops beside its work loop (the helper
+    wave's in search_duo): the depth-1 loop whose head holds the first atomic add."""
+    at = [k for k, x in enumerate(body) if "global_atomic_add" in x]
+    if not at:
+        return False
+    d1 = [k for k in range(at[0]) if "This Loop Header: Depth=1" in body[k]]
+    if not d1:
+        return False
+    seg = "\n".join(body[at[0]:at[0] + 25])
+    # the exit compare: scalar, or a vector compare of the broadcast position with the count
+    # (loaded into a vector register, equal in every lane) and a branch on vcc
+    scalar_exit = "s_cmp_ge_u32" in seg
+    vector_exit = re.search(r"v_cmp_ge_u32_e32 vcc, s\d+, v\d+", seg) is not None and "s_cbranch_vccnz" in seg
+    if "v_readfirstlane_b32" not in seg or not (scalar_exit or vector_exit):
+        return False
+    return "s_andn2_b64 exec, exec" not in "\n".join(body[max(0, d1[-1] - 8):d1[-1]])
+
+
 def test_persistent_loop_is_wave_uniform(search_asm):
     kernels = {}
     for text in search_asm:
         kernels.update(_functions(text, r"_ZN4song17search_persistent"))
-    assert len(kernels) == 60
+    # 3 metrics x (exact, fp32) x 3 row storages x 5 row shapes, and the LDS-bag form
+    # (float32 rows) of each metric, mode and shape
+    assert len(kernels) == 90
     bad = [k for k, body in kernels.items() if not loop_head_is_uniform(body)]
     assert not bad, f"{len(bad)} persistent kernels with a split work loop, e.g. {bad[:3]}"
+
+
+def test_two_wave_kernel_work_loop_is_wave_uniform(search_asm):
+    kernels = {}
+    for text in search_asm:
+        kernels.update(_functions(text, r"_ZN4song10search_duo"))
+    assert len(kernels) == 15  # 3 metrics x 5 row shapes (FP32 mode, float32 rows)
+    bad = [k for k, body in kernels.items() if not work_loop_is_uniform(body)]
+    assert not bad, f"{len(bad)} two-wave kernels with a split work loop, e.g. {bad[:3]}"
 
 
 def test_checker_flags_a_split_loop():
@@ -90,3 +120,6 @@ def test_checker_flags_a_split_loop():
     assert loop_head_is_uniform(good)
     split = ["s_andn2_b64 exec, exec, s[0:1]", "s_cbranch_execz .LBB0_298"] + good[1:3] + ["s_nop 0"] * 30 + good[3:]
     assert not loop_head_is_uniform(split)
+    assert work_loop_is_uniform(["s_cbranch_scc1 .LBB0_9", ".LBB0_2:  ; Inner Loop Header: Depth=1", "s_sleep 2",
+                                 "s_branch .LBB0_2"] + good)
+    assert not work_loop_is_uniform(["s_cbranch_scc1 .LBB0_9"] + split)
