@@ -268,6 +268,11 @@ __global__ __launch_bounds__(64) void features_kernel(const float *Q, int dim, c
 // CUs at one wave per SIMD, and the output layer waited on its weight loads)
 constexpr int IQ = 32;
 constexpr int MI = IQ / 16;
+// waves per workgroup: they split the output layer's columns (64 per wave and chunk) and the
+// hidden units (slices of 64); 8 waves per group, each with half the chain, measured slower (279
+// against 256 us for 10k queries)
+constexpr int IW = 4;
+constexpr int IT = 64 * IW;  // threads per workgroup
 
 struct InferArgs {
     const uint16_t *X;
@@ -429,7 +434,7 @@ __device__ __forceinline__ Soft soft_merge(const Soft &x, const Soft &y) {
     return t;
 }
 
-__global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
+__global__ __launch_bounds__(IT) void infer_kernel(InferArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r0 = blockIdx.x * IQ;
@@ -437,26 +442,26 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
     uint16_t *X = reinterpret_cast<uint16_t *>(smem);
     uint16_t *H1 = X + IQ * ldx;
     uint16_t *H2 = H1 + IQ * ldh;
-    Soft *part = reinterpret_cast<Soft *>(H2 + IQ * ldh);  // [4 waves][IQ]
-    Soft *run = part + 4 * IQ;                              // [IQ]
+    Soft *part = reinterpret_cast<Soft *>(H2 + IQ * ldh);  // [IW waves][IQ]
+    Soft *run = part + IW * IQ;                             // [IQ]
     float *stp = reinterpret_cast<float *>(run + IQ);       // [IQ]
 
-    for (int e = tid; e < IQ * a.width; e += 256) {
+    for (int e = tid; e < IQ * a.width; e += IT) {
         const int r = e / a.width, c = e % a.width;
         X[r * ldx + c] = (r0 + r < a.batch) ? a.X[(long long)(r0 + r) * a.width + c] : (uint16_t)0;
     }
-    for (int r = tid; r < IQ; r += 256) run[r] = soft_empty();
+    for (int r = tid; r < IQ; r += IT) run[r] = soft_empty();
     __syncthreads();
 
     v4f acc[MI][4];
-    // hidden layers: the 4 waves split the hidden units in slices of 64
+    // hidden layers: the waves split the hidden units in slices of 64
     for (int layer = 0; layer < 2; ++layer) {
         const uint16_t *in = layer == 0 ? X : H1;
         const int ldin = layer == 0 ? ldx : ldh, K = layer == 0 ? a.width : a.hidden;
         const uint16_t *W = layer == 0 ? a.W1 : a.W2;
         const float *bias = layer == 0 ? a.b1 : a.b2;
         uint16_t *out = layer == 0 ? H1 : H2;
-        for (int n0 = wave * 64; n0 < a.hidden; n0 += 256) {
+        for (int n0 = wave * 64; n0 < a.hidden; n0 += IT) {
             tile_layer(in, ldin, K, W, a.hidden, n0, acc);
 #pragma unroll
             for (int mi = 0; mi < MI; ++mi)
@@ -471,16 +476,16 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
         __syncthreads();
     }
 
-    // output layer in chunks of 256 columns: each lane keeps a running softmax state (top two
-    // and the sum) for each of its 16 rows over the columns it owns; the 16 lanes of a row and
-    // the 4 waves combine once at the end
+    // output layer in chunks of 64 columns per wave: each lane keeps a running softmax state (top
+    // two and the sum) for each of its rows over the columns it owns; the 16 lanes of a row and
+    // the waves combine once at the end
     const int NO = a.C + 1;
     Soft ts[MI][4];
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
         for (int j = 0; j < 4; ++j) ts[mi][j] = soft_empty();
-    for (int c0 = 0; c0 < NO; c0 += 256) {
+    for (int c0 = 0; c0 < NO; c0 += IT) {
         const int n0 = c0 + wave * 64;
         tile_layer_il(H2, ldh, a.hidden, a.W3i, NO, a.npad, n0, acc);
 #pragma unroll
@@ -525,7 +530,7 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
     __syncthreads();
     if (tid < IQ) {
         Soft t = soft_empty();
-        for (int w = 0; w < 4; ++w) t = soft_merge(t, part[w * IQ + tid]);
+        for (int w = 0; w < IW; ++w) t = soft_merge(t, part[w * IQ + tid]);
         run[tid] = t;
     }
     __syncthreads();
@@ -533,7 +538,7 @@ __global__ __launch_bounds__(256) void infer_kernel(InferArgs a) {
     // the last hidden layer by TPR threads per row (entries e = quarter, quarter + TPR, ...);
     // without neighbourhoods, the mass of the top two
     {
-        constexpr int TPR = 256 / IQ;
+        constexpr int TPR = IT / IQ;
         const int row = tid / TPR, quarter = tid % TPR;
         const Soft t = run[row];
         float mass = 0.0f;
@@ -817,9 +822,10 @@ extern "C" int song_mlp_infer_nbr(song_mlp *m, const uint16_t *d_x, int batch, c
     a.logits = d_logits;
     a.nbr = d_nbr;
     a.n_nbr = n_nbr;
-    const size_t lds = (size_t)IQ * (m->width + 8) * 2 + 2 * (size_t)IQ * (m->hidden + 8) * 2 + (5 * IQ) * sizeof(Soft) + IQ * 4;
+    const size_t lds = (size_t)IQ * (m->width + 8) * 2 + 2 * (size_t)IQ * (m->hidden + 8) * 2 +
+                       ((IW + 1) * IQ) * sizeof(Soft) + IQ * 4;
     if (lds > 65536) MHIP(hipFuncSetAttribute((const void *)infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(infer_kernel, dim3((batch + IQ - 1) / IQ), dim3(256), lds, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(infer_kernel, dim3((batch + IQ - 1) / IQ), dim3(IT), lds, (hipStream_t)stream, a);
     MHIP(hipGetLastError());
     return 0;
 }
