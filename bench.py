@@ -405,7 +405,9 @@ def bench_c2(args, torch, song_amd):
                      "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK, 5),
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "kernel": "search_kernel<L2, fp32, F32 rows, shape 1> (plain form, one launch per step)",
+                     "kernel": ("search_kernel<L2, fp32, F32 rows, shape 1> (plain form, one launch per step; "
+                                "a call of more queries than fit the device at once keeps the best lists' "
+                                "fronts in global memory)"),
                      "kernel_ms": round(ms_plain, 4),
                      "algorithmic_bytes_per_launch": alg_plain,
                      "learned_achieved_GBps": (round(learned_info["algorithmic_bytes_per_step"]

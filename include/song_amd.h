@@ -85,6 +85,13 @@ typedef struct {
                       sized by the reference's auto_tune_bloom closed form for pq); -1 = size and
                       bits both from that closed form.  Approximate: a false positive skips an
                       unvisited vertex, so ids may differ from the exact search (recall-judged) */
+    int best_in_global; /* where the pq-best list's front (all but its last 64 entries) lives: 1 = the
+                           query slot's global scratch, after its queue bag (frees pq keys of LDS per
+                           wave, so more waves fit a CU; float32 rows), 0 = LDS, -1 = by the call's
+                           size (default): global when its queries outnumber the waves that fit the
+                           device at once with the front in LDS, else LDS (a call that fits at once
+                           is latency-bound, and the front's global stores lengthen each step);
+                           results do not depend on it */
 } song_search_opts;
 
 /* Fill opts with the defaults above. */
@@ -209,6 +216,10 @@ typedef struct {
                             ahead of the search, the distances of the predicted next expansion's
                             unvisited neighbours into an LDS cache the search wave reads (same results;
                             FP32 mode, float32 rows, exact visited table; waves_per_group is then 1) */
+    int best_in_global;  /* as song_search_opts.best_in_global, but -1 (the default) = 0, LDS: a tier's
+                            share of the batch is not known when the batch is created, and the autotuner
+                            sets 1 where it measures faster (one query per wave, float32 rows, bag in
+                            global memory; the re-run keeps the front in LDS); same results */
 } song_tier;
 
 typedef struct {

@@ -36,14 +36,16 @@ class QueryParams(ctypes.Structure):
 class SearchOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("slack", ctypes.c_int), ("hash_log2", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("retry", ctypes.c_int), ("storage", ctypes.c_int),
-                ("rerank", ctypes.c_int), ("visited_bits", ctypes.c_int), ("bloom_k", ctypes.c_int)]
+                ("rerank", ctypes.c_int), ("visited_bits", ctypes.c_int), ("bloom_k", ctypes.c_int),
+                ("best_in_global", ctypes.c_int)]
 
 
 class Tier(ctypes.Structure):
     _fields_ = [("pq_max", ctypes.c_int), ("hash_log2", ctypes.c_int), ("slack", ctypes.c_int), ("shape", ctypes.c_int),
                 ("waves_per_group", ctypes.c_int), ("groups_per_cu", ctypes.c_int), ("visited_bits", ctypes.c_int),
                 ("storage", ctypes.c_int), ("bloom_k", ctypes.c_int), ("queries_per_wave", ctypes.c_int),
-                ("multiprobe", ctypes.c_int), ("bag_in_lds", ctypes.c_int), ("waves_per_query", ctypes.c_int)]
+                ("multiprobe", ctypes.c_int), ("bag_in_lds", ctypes.c_int), ("waves_per_query", ctypes.c_int),
+                ("best_in_global", ctypes.c_int)]
 
 
 MAX_TIERS = 8
@@ -161,11 +163,12 @@ def metric_code(metric):
 
 
 def make_opts(mode=MODE_AUTO, slack=128, hash_log2=0, max_steps=0, retry=1, storage=STORE_F32, rerank=1,
-              visited_bits=0, bloom_k=0):
+              visited_bits=0, bloom_k=0, best_in_global=-1):
     o = SearchOpts()
     lib().song_search_opts_default(ctypes.byref(o))
     o.mode, o.slack, o.hash_log2, o.max_steps, o.retry = mode, slack, hash_log2, max_steps, retry
     o.storage, o.rerank, o.visited_bits, o.bloom_k = storage, rerank, visited_bits, bloom_k
+    o.best_in_global = best_in_global
     return o
 
 
@@ -219,7 +222,7 @@ class Index:
         return codes, scales
 
     def search(self, queries, pq, k=None, mode=MODE_AUTO, slack=128, hash_log2=0, retry=1, storage=STORE_F32,
-               rerank=1, visited_bits=0, bloom_k=0):
+               rerank=1, visited_bits=0, bloom_k=0, best_in_global=-1):
         """Host buffers in and out: ids int64 [nq, k] (-1 padding), dists float32, flags uint32."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
@@ -227,7 +230,7 @@ class Index:
         ids = np.empty((nq, k), dtype=np.uint64)
         ds = np.empty((nq, k), dtype=np.float32)
         fl = np.empty(nq, dtype=np.uint32)
-        o = make_opts(mode, slack, hash_log2, 0, retry, storage, rerank, visited_bits, bloom_k)
+        o = make_opts(mode, slack, hash_log2, 0, retry, storage, rerank, visited_bits, bloom_k, best_in_global)
         _check(lib().song_search(self._h, _ptr(q), nq, pq, k, _ptr(ids), _ptr(ds), _ptr(fl), ctypes.byref(o)))
         out = ids.astype(np.int64)
         out[ids == np.uint64(0xFFFFFFFFFFFFFFFF)] = -1
@@ -235,7 +238,7 @@ class Index:
 
     def search_device(self, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
                       mode=MODE_AUTO, slack=128, hash_log2=0, retry=1, stream=None, storage=STORE_F32, rerank=1,
-                      max_steps=0, entry=None, steps=None, subset=None, visited_bits=0, bloom_k=0):
+                      max_steps=0, entry=None, steps=None, subset=None, visited_bits=0, bloom_k=0, best_in_global=-1):
         """Device buffers (torch tensors on cuda): queries float32 [nq, dim], ids int32/uint32 [nq, >=k].
 
         entry: int32 [nq] or [nq, E] start vertices (learned form; default vertex 0);
@@ -262,7 +265,7 @@ class Index:
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
-        o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank, visited_bits, bloom_k)
+        o = make_opts(mode, slack, hash_log2, max_steps, retry, storage, rerank, visited_bits, bloom_k, best_in_global)
         qp = QueryParams()
         qp.pq, qp.k = (pq_arr.data_ptr() if pq_arr is not None else None), (k_arr.data_ptr() if k_arr is not None else None)
         qp.entry = entry.data_ptr() if entry is not None else None
@@ -277,7 +280,8 @@ class Index:
 
 def search_device_noalloc(index, queries, pq, k, ids, dists=None, flags=None, stats=None, pq_arr=None, k_arr=None,
                           mode=MODE_EXACT, slack=128, hash_log2=0, storage=STORE_F32, rerank=1, max_steps=0,
-                          entry=None, steps=None, subset=None, visited_bits=0, stream=None, bloom_k=0):
+                          entry=None, steps=None, subset=None, visited_bits=0, stream=None, bloom_k=0,
+                          best_in_global=-1):
     """song_search_device_noalloc: Index.search_device without device allocation (scratch in the
     library's static per-GPU array); mode must be MODE_EXACT or MODE_FP32, overflowed queries keep
     their flags (no re-run).  Argument checks as search_device; no host synchronization."""
@@ -297,7 +301,7 @@ def search_device_noalloc(index, queries, pq, k, ids, dists=None, flags=None, st
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
-    o = make_opts(mode, slack, hash_log2, max_steps, 0, storage, rerank, visited_bits, bloom_k)
+    o = make_opts(mode, slack, hash_log2, max_steps, 0, storage, rerank, visited_bits, bloom_k, best_in_global)
     qp = _query_params(pq_arr, k_arr, entry, steps, subset)
     _check(lib().song_search_device_noalloc(index._h, dp(queries), nq, pq, k, ctypes.byref(qp), dp(ids), dp(dists),
                                             ids.shape[1], dp(flags), dp(stats), ctypes.byref(o),
@@ -329,21 +333,23 @@ def resolve_mode(index, queries, stream=None):
 
 
 TIER_KEYS = ("pq_max", "hash_log2", "slack", "shape", "waves_per_group", "groups_per_cu", "visited_bits", "storage",
-             "bloom_k", "queries_per_wave", "multiprobe", "bag_in_lds", "waves_per_query")
+             "bloom_k", "queries_per_wave", "multiprobe", "bag_in_lds", "waves_per_query", "best_in_global")
 
 
 def tier(pq_max, hash_log2=0, slack=32, shape=-1, waves_per_group=1, groups_per_cu=0, visited_bits=0, storage=-1,
-         bloom_k=0, queries_per_wave=1, multiprobe=1, bag_in_lds=0, waves_per_query=1):
+         bloom_k=0, queries_per_wave=1, multiprobe=1, bag_in_lds=0, waves_per_query=1, best_in_global=-1):
     """One tier of a batch configuration (see song_tier in include/song_amd.h); storage -1 = the
     batch's row storage; bloom_k != 0: a Bloom filter in place of the exact visited table;
     queries_per_wave 2 / 4: sub-groups of 32 / 16 lanes each searching its own query; multiprobe 2
     (with queries_per_wave > 1): two expansions per step (approximate); bag_in_lds 1 (one query
     per wave, float32 rows): the queue's overflow bag in LDS; waves_per_query 2: a helper wave per query
-    computes the predicted next expansion's distances ahead (FP32 mode, float32 rows)."""
+    computes the predicted next expansion's distances ahead (FP32 mode, float32 rows);
+    best_in_global 1 / 0: the best list's front in the slot's global scratch / in LDS (one query per
+    wave, float32 rows, bag in global memory); -1 = 0 for a tier."""
     return dict(pq_max=pq_max, hash_log2=hash_log2, slack=slack, shape=shape, waves_per_group=waves_per_group,
                 groups_per_cu=groups_per_cu, visited_bits=visited_bits, storage=storage, bloom_k=bloom_k,
                 queries_per_wave=queries_per_wave, multiprobe=multiprobe, bag_in_lds=bag_in_lds,
-                waves_per_query=waves_per_query)
+                waves_per_query=waves_per_query, best_in_global=best_in_global)
 
 
 class Batch:

@@ -21,6 +21,9 @@ coordinate descent over the knobs of a tier (song_tier in include/song_amd.h):
   bag_in_lds       one query per wave on float32 rows: the queue's overflow bag in LDS (same results)
   waves_per_query  1, or 2: a helper wave per query computes the predicted next expansion's distances
                    ahead of the search wave (FP32 mode, float32 rows; same results)
+  best_in_global   the best list's front in the slot's global scratch (1), in LDS (0), or by size (-1):
+                   global frees pq keys of LDS per wave for throughput-bound tiers; a tier of few
+                   queries is faster with it in LDS (same results)
 
 Each candidate runs the tier's queries through song_batch_run and is timed with events
 (median of `reps`); a candidate is accepted only if every result equals the default
@@ -44,12 +47,21 @@ def default_tiers(pqs):
     return [tier(int(p)) for p in sorted(set(int(x) for x in pqs))]
 
 
-def _time_run(batch, q, ids, flags, pq_arr, k_arr, dists, reps, entry=None, steps=None):
+def _time_run(batch, q, ids, flags, pq_arr, k_arr, dists, reps, entry=None, steps=None, min_ms=30.0, max_reps=15):
+    """Median of at least `reps` timed runs, more (up to max_reps) for short runs, so that a tier of
+    a few milliseconds is timed over about min_ms."""
     import torch
 
     batch.run(q, ids, flags, pq_arr=pq_arr, k_arr=k_arr, dists=dists, entry=entry, steps=steps)  # warm-up
-    ts = []
-    for _ in range(reps):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    batch.run(q, ids, flags, pq_arr=pq_arr, k_arr=k_arr, dists=dists, entry=entry, steps=steps)
+    e1.record()
+    e1.synchronize()
+    first = e0.elapsed_time(e1)
+    reps = int(min(max_reps, max(reps, np.ceil(min_ms / max(first, 1e-3)))))
+    ts = [first]
+    for _ in range(reps - 1):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         batch.run(q, ids, flags, pq_arr=pq_arr, k_arr=k_arr, dists=dists, entry=entry, steps=steps)
@@ -131,7 +143,8 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                  ("slack", (0, 8, 32, 128)),
                  ("groups_per_cu", (0, 1, 2, 4, 8)),
                  ("visited_bits", (0, 32)),
-                 ("bag_in_lds", (0, 1))]
+                 ("bag_in_lds", (0, 1)),
+                 ("best_in_global", (-1, 0, 1))]
         if storage != STORE_F32:
             knobs.append(("storage", (-1, STORE_F32)))
         if judge is not None:
@@ -144,6 +157,7 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
         best.setdefault("multiprobe", 1)
         best.setdefault("bag_in_lds", 0)
         best.setdefault("waves_per_query", 1)
+        best.setdefault("best_in_global", -1)
         rows_f32 = storage == STORE_F32 or best.get("storage", -1) == STORE_F32
         for _round in range(2):
             for key, choices in knobs:
@@ -164,7 +178,11 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                     continue
                 if key == "multiprobe" and best["queries_per_wave"] == 1:
                     continue
-                if key == "bag_in_lds" and (best["queries_per_wave"] > 1 or not rows_f32 or best["waves_per_query"] > 1):
+                if key == "bag_in_lds" and (best["queries_per_wave"] > 1 or not rows_f32 or best["waves_per_query"] > 1
+                                            or best["best_in_global"] == 1):
+                    continue
+                # the best list's front in global memory: one query per wave, float32 rows, global bag
+                if key == "best_in_global" and (best["queries_per_wave"] > 1 or not rows_f32 or best["bag_in_lds"]):
                     continue
                 # the helper wave: FP32 mode, float32 rows, exact table, one query per wave, global bag
                 if key == "waves_per_query" and (mode == MODE_EXACT or not rows_f32 or bloom or best["queries_per_wave"] > 1
@@ -183,6 +201,7 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                         cand["multiprobe"] = 1
                     if key == "queries_per_wave" and v > 1:
                         cand["bag_in_lds"] = 0
+                        cand["best_in_global"] = -1
                     if key == "shape" and cand["queries_per_wave"] > 1 and v > 3:
                         continue
                     approx = (key in ("bloom_k", "multiprobe") or (key == "hash_log2" and bloom)
@@ -193,9 +212,10 @@ def tune(index, queries, pq_arr, k_arr, tiers=None, mode=MODE_EXACT, storage=STO
                         cand["waves_per_group"] = 1
                         ms, out = measure(cand, want=ref, approx=approx)
                     trials.append((dict(cand), ms))
-                    # a grid smaller than all that fit must win clearly: it caps the queries in
-                    # flight, which costs a batch larger than the tuning sample
-                    margin = 0.97 if key == "groups_per_cu" and v != 0 else 1.0
+                    # a candidate must be 1% faster (timing noise); a grid smaller than all that
+                    # fit must win clearly: it caps the queries in flight, which costs a batch
+                    # larger than the tuning sample
+                    margin = 0.97 if key == "groups_per_cu" and v != 0 else 0.99
                     if ms is not None and ms < best_ms * margin:
                         best, best_ms = cand, ms
                         if approx:

@@ -137,6 +137,7 @@ extern "C" void song_batch_config_default(song_batch_config *c, int pq_max) {
     t.multiprobe = 1;
     t.bag_in_lds = 0;
     t.waves_per_query = 1;
+    t.best_in_global = -1;  // = 0: a tier's share of the batch is not known when it is created
     c->tier[0] = t;
     c->retry = t;
     c->retry.hash_log2 = 0;  // the largest tier's filter x 4
@@ -257,7 +258,7 @@ extern "C" int song_batch_create(song_index *ix, const song_batch_config *cfg, u
         const int bagl = is_retry ? 0 : ti.bag_in_lds;
         const int wpq = is_retry ? 1 : std::max(1, ti.waves_per_query);
         int rc = make_plan(ix, exact, store, ti.pq_max, ti.slack, hlog, &L.plan, ti.shape, ti.visited_bits, ti.bloom_k,
-                           qpw, bagl, wpq);
+                           qpw, bagl, wpq, (is_retry || ti.best_in_global < 0) ? 0 : ti.best_in_global);
         // the re-run's default table (the largest tier's, 4x) may not fit one wave's LDS: take the
         // largest that does (the re-run also has a 4x larger global spill table)
         while (rc == SONG_E_OVERFLOW && is_retry && !ti.hash_log2 && hlog > top_hlog) {

@@ -35,6 +35,7 @@ struct song_index {
     int integer_valued = 0;
     float vmin = 0.f, vmax = 0.f;
     size_t max_lds = 65536;
+    int n_cu = 1;  // compute units of the index's device
     // visited spill tables and queue bags, one per launch slot, one set per stream: calls
     // on one stream run in order; calls on different streams use different buffers
     struct Spill {
@@ -61,11 +62,15 @@ struct Plan {
     int lanes;   // lanes per query: 64 (one query per wave, search_query) or 16 / 32 (search_group)
     int bag_lds; // search_query's bag in LDS (qcap keys after the query) instead of global memory
     int duo;     // two waves per query (search_duo: search wave + helper wave)
+    int pq;      // budget the layout is sized for
+    int bglob;   // search_query's best-list front in global memory, after the slot's bag
     size_t lds;  // LDS bytes of one wave (lanes 64) or of one query's slice (lanes 16 / 32)
 };
 
-size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k = 0, int bag_lds = 0);
-// bytes of one wave slot's queue bag (qcap keys, 256-byte aligned)
+size_t lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k = 0, int bag_lds = 0,
+                 int bglob = 0);
+// bytes of one wave slot's global region: the queue bag (qcap keys, 256-byte aligned), then
+// with bglob the best list's front (pq keys)
 size_t bag_stride(const Plan &p);
 // shape < 0: by row width; otherwise the given lanes-per-row shape
 // visited_bits: 16 or 32 = slot width (16 only where the ids fit, else 32); 0 = 16 where they fit
@@ -76,9 +81,14 @@ size_t bag_stride(const Plan &p);
 // bag_lds: the queue's bag in LDS (one query per wave, float32 rows, persistent launches)
 // waves_per_query: 1, or 2 (search_duo: float32 mode, float32 rows, exact visited table, bag in global
 // memory, persistent launches)
+// best_in_global: 1 = the best list's front in the slot's global region (one query per wave, float32
+// rows, bag in global memory), 0 = in LDS, -1 = by the launch's size: in global memory where allowed
+// and the launch's nq_hint queries (0: unknown) outnumber the waves that fit the device at once with
+// the front in LDS -- more waves per CU then pay for the front's global traffic; a launch that fits
+// at once is latency-bound and keeps the front in LDS (same results either way)
 int make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape = -1,
               int visited_bits = 0, int bloom_k = 0, int queries_per_wave = 1, int bag_lds = 0,
-              int waves_per_query = 1);
+              int waves_per_query = 1, int best_in_global = -1, uint64_t nq_hint = 0);
 // LDS bytes of one query's slice in search_group (its layout: visited | queue | best | stage | cid | qv)
 size_t group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, int vis16, int lanes);
 int default_hlog(int pq);

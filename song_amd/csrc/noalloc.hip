@@ -100,7 +100,8 @@ extern "C" int song_search_device_noalloc(song_index *ix, const float *d_queries
     hlog = std::max(8, std::min(hlog, 16));
     if (opts.bloom_k != 0) hlog = opts.hash_log2 > 0 ? std::min(opts.hash_log2, 15) : 0;
     Plan plan;
-    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan, -1, opts.visited_bits, opts.bloom_k);
+    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan, -1, opts.visited_bits, opts.bloom_k, 1, 0,
+                       1, opts.best_in_global, nrun);
     if (rc != SONG_OK) return rc;
 
     int dev = 0;

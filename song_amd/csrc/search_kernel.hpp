@@ -23,13 +23,18 @@
 //              approximate, a blocked Bloom filter of 2^hlog 32-bit words (bloom_insert)
 //   scratch  : 64 keys    merge staging of the queue head (the head itself is one key per
 //                          lane in registers, ascending over lanes [h, h + nq), nq <= 64)
-//   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list)
+//   best     : pq keys, sorted ascending, first nbest live (pq-best expanded list); absent
+//              with bglob, where the list's front lives in the wave slot's global region
 //   cid      : 256 u32   candidate ids of the current expansion / radix-select bins
 //   qv       : qld f32   the query, zero-padded to a multiple of 16
 // The bag (qcap = pq + slack + 128 keys, unsorted, first bn live) is in global memory, one
 // region per wave slot: it is written by plain stores as candidates arrive and read only when
 // the head runs empty or the bag nears its capacity, so LDS holds only what every step
-// touches and more waves fit on a CU.
+// touches and more waves fit on a CU.  The best list's last 64 positions are one key per lane
+// in registers; its front (the first nbest - 64) is written once per entry while the list
+// grows and read only when an expansion lands below the register tail, which is rare, and at
+// the end -- so with bglob it lives after the bag in the slot's global region (pq keys) and a
+// large pq does not cost LDS.
 //
 // Each wave owns its LDS slice (one wave per workgroup in search_kernel; 1-4 waves, each
 // on its own slice, in search_persistent).  LDS instructions of a wave execute in issue
@@ -103,6 +108,11 @@ struct SearchParams {
     int multiprobe;         // several-queries-per-wave kernel: expansions per step (1, or 2: the next
                             // queue head too; approximate)
 };
+
+// bytes of a slot's bag before the best-list front (bglob): qcap keys rounded to 256
+template <class Key> __device__ __forceinline__ size_t bag_bytes(int qcap) {
+    return ((size_t)qcap * sizeof(Key) + 255) & ~(size_t)255;
+}
 
 // ---------------------------------------------------------------- keys
 
@@ -794,7 +804,9 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin
 // global memory -- for latency-bound launches with few waves per CU, where a refill or a
 // truncation then scans LDS instead of waiting on global-memory round trips.
 // HELP: the search wave of search_duo (float32 keys); duo = the mailbox and cache in LDS.
-template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false, bool HELP = false>
+// BG: the best list's front in the slot's global region after the bag (float32 rows, bag in
+// global memory); the launch sizes each slot's global region for it (bag_bytes + pq keys).
+template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false, bool HELP = false, bool BG = false>
 __device__ __forceinline__ void search_query(const SearchParams &P, const int qi, const int slot_q,
                                              unsigned char *smem, unsigned char *duo = nullptr) {
     using Key = typename KeySel<EXACT>::T;
@@ -814,8 +826,24 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     const uint32_t vis_bytes = bloom ? (4u << P.hlog) : H * (P.vis16 ? 2u : 4u);
     const int gbits = P.hlog - 3;                               // compact table: groups of 8
     Key *S = reinterpret_cast<Key *>(smem + vis_bytes);        // merge scratch, QH keys
+    // the best list's front: LDS after the scratch, or (bglob: float32 rows, global bag) the
+    // slot's global region after its bag
+    static_assert(!BG || (STORE == ST_F32 && !BAGL), "the global front needs float32 rows and a global bag");
+    constexpr bool bglob = BG;
     Key *B = S + QH;
-    uint32_t *cid = reinterpret_cast<uint32_t *>(B + ((P.pq + 1) & ~1));  // 256 u32
+    // the front (computed where used, in rare paths, so that no registers hold a global
+    // pointer through the loop)
+    const auto Bf = [&]() -> Key * {
+        if constexpr (BG)
+            return reinterpret_cast<Key *>(P.gbag + (size_t)slot_q * P.gbag_stride + bag_bytes<Key>(P.qcap));
+        else
+            return B;
+    };
+    // global front: its stores are complete before it is read
+    const auto front_fence = [&]() {
+        if constexpr (BG) bag_fence();
+    };
+    uint32_t *cid = reinterpret_cast<uint32_t *>(B + (bglob ? 0 : ((P.pq + 1) & ~1)));  // 256 u32
     float *qv = reinterpret_cast<float *>(cid + 256);
     // bag, qcap keys: global memory (one region per slot), or this wave's LDS after the query
     Key *Bg = BAGL ? reinterpret_cast<Key *>(qv + P.qld)
@@ -824,7 +852,9 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
     // and touch neither (a host / kernel mismatch becomes an error, not out-of-range accesses)
     const unsigned char *lds_end = BAGL ? reinterpret_cast<unsigned char *>(Bg + P.qcap)
                                         : reinterpret_cast<unsigned char *>(qv + P.qld);
-    if (lds_end - smem > (ptrdiff_t)P.wave_lds || (!BAGL && (!P.gbag || (size_t)P.qcap * sizeof(Key) > P.gbag_stride))) {
+    const size_t gneed = bglob ? bag_bytes<Key>(P.qcap) + (size_t)P.pq * sizeof(Key) : (size_t)P.qcap * sizeof(Key);
+    if (lds_end - smem > (ptrdiff_t)P.wave_lds ||
+        (!BAGL && (!P.gbag || gneed > P.gbag_stride))) {
         if (lane == 0 && P.out_flags) P.out_flags[qi] = F_BAD_LAUNCH;
         return;
     }
@@ -1002,7 +1032,7 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
             const uint64_t below = ballot(lt(bt, now));
             const bool in_tail = tb == 0 || (below & 1ull);
             const int p = in_tail ? tb + __popcll(below)
-                                  : prefix_count(B, tb, [&](const Key &e) { return lt(e, now); });
+                                  : (front_fence(), prefix_count(Bf(), tb, [&](const Key &e) { return lt(e, now); }));
             if (p < k && lane == 0) settle = steps;
             if (p < pq) {
                 const int j = p - tb;  // tail lane of the new entry; negative: it lands in the front
@@ -1011,12 +1041,12 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                     if (j >= 0) {
                         // the 65 entries [tb, nbest] with `now` at j: the first goes to the front,
                         // lanes below j - 1 take their upper neighbour's entry
-                        if (lane == 0) B[tb] = ksel(j == 0, now, bt);
+                        if (lane == 0) Bf()[tb] = ksel(j == 0, now, bt);
                         const Key up = shl1(bt);
                         bt = ksel(lane < j - 1, up, ksel(lane == j - 1, now, bt));
                     } else {
                         // front [p, tb) moves to [p + 1, tb + 1); the tail keeps its entries
-                        shift_insert(B, p, tb, now);
+                        shift_insert(Bf(), p, tb, now);  // (a front in global memory: after the fence above)
                     }
                     ++tb;
                 } else if (j >= 0) {
@@ -1025,8 +1055,8 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
                     bt = ksel(lane < j, bt, ksel(lane == j, now, dn));
                 } else {
                     // full, `now` in the front: the front's last entry moves into the tail
-                    const Key fl = B[tb - 1];
-                    shift_insert(B, p, tb - 1, now);
+                    const Key fl = Bf()[tb - 1];
+                    shift_insert(Bf(), p, tb - 1, now);
                     const Key dn = shr1(bt);
                     bt = ksel(lane == 0, fl, dn);
                 }
@@ -1187,9 +1217,11 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
         if (flags != 0u || expand(std::integral_constant<int, 0>{})) break;
         if (expand(std::integral_constant<int, 1>{})) break;
     }
-    // the whole best list in LDS from here on
+    // the whole best list in LDS (or, bglob, in the slot's global region) from here on
+    if constexpr (BG) B = Bf();
     if (lane < nbest - tb) B[tb + lane] = bt;
-    lds_fence();
+    if constexpr (BG) bag_fence();
+    else lds_fence();
 
     const int cnt = min(k, nbest);
     const size_t ob = (size_t)qi * (size_t)P.out_ld;
@@ -1253,18 +1285,18 @@ __device__ __forceinline__ void search_query(const SearchParams &P, const int qi
 }
 
 // One workgroup (one wave) per query of the launch.
-template <int METRIC, bool EXACT, int STORE, int SHAPE>
+template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BG = false>
 __global__ __launch_bounds__(64) void search_kernel(SearchParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int slot_q = (int)blockIdx.x;
     const int qi = P.qidx ? (int)P.qidx[P.qbase + slot_q] : P.qbase + slot_q;
-    search_query<METRIC, EXACT, STORE, SHAPE>(P, qi, slot_q, smem);
+    search_query<METRIC, EXACT, STORE, SHAPE, false, false, BG>(P, qi, slot_q, smem);
 }
 
 // Persistent grid: every wave takes queries from the shared list until it is exhausted
 // (each wave reaches the exit once *work_next passes *work_count).  A workgroup holds
 // blockDim.x / 64 waves, each on its own wave_lds-byte slice of LDS and its own spill table.
-template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false>
+template <int METRIC, bool EXACT, int STORE, int SHAPE, bool BAGL = false, bool BG = false>
 __global__ __launch_bounds__(256) void search_persistent(SearchParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int w = (int)(threadIdx.x >> 6);
@@ -1283,24 +1315,24 @@ __global__ __launch_bounds__(256) void search_persistent(SearchParams P) {
         const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
         if (t >= count) break;
         const int qi = P.qidx ? (int)P.qidx[t] : (int)t;
-        search_query<METRIC, EXACT, STORE, SHAPE, BAGL>(P, qi, slot, mine);
+        search_query<METRIC, EXACT, STORE, SHAPE, BAGL, false, BG>(P, qi, slot, mine);
     }
 }
 
 // The helper wave of search_duo (see DuoMail): slice = the search wave's LDS slice.
-template <int METRIC, int STORE, int SHAPE>
+template <int METRIC, int STORE, int SHAPE, bool BG>
 __device__ void duo_helper(const SearchParams &P, unsigned char *slice, unsigned char *duo) {
     constexpr int SUB = Shape<SHAPE>::SUB, NV = Shape<SHAPE>::NV;
     const int lane = lane_id();
     DuoMail *mail = reinterpret_cast<DuoMail *>(duo);
     uint64_t *dcache = reinterpret_cast<uint64_t *>(duo + sizeof(DuoMail));
     uint32_t *hid = reinterpret_cast<uint32_t *>(dcache + DUO_CACHE);
-    // the search wave's layout (search_query): visited | 64 keys | pq keys | 256 u32 | qv
+    // the search wave's layout (search_query): visited | 64 keys | pq keys (not with bglob) | 256 u32 | qv
     const uint32_t H = 1u << P.hlog;
     const uint32_t vis_bytes = H * (P.vis16 ? 2u : 4u);
     const uint32_t *vis = reinterpret_cast<const uint32_t *>(slice);
     const KeyF *S = reinterpret_cast<const KeyF *>(slice + vis_bytes);
-    const uint32_t *cid = reinterpret_cast<const uint32_t *>(S + 64 + ((P.pq + 1) & ~1));
+    const uint32_t *cid = reinterpret_cast<const uint32_t *>(S + 64 + (BG ? 0 : ((P.pq + 1) & ~1)));
     const float *qv = reinterpret_cast<const float *>(cid + 256);
     uint32_t my_epoch = 0u, last_t = 0u;
     float lenv = 0.0f;
@@ -1343,7 +1375,7 @@ __device__ void duo_helper(const SearchParams &P, unsigned char *slice, unsigned
 // Two waves per query, one query per workgroup at a time: wave 0 searches (the persistent work
 // loop of search_persistent), wave 1 helps (duo_helper).  LDS: the search wave's wave_lds slice,
 // then DUO_BYTES of mailbox, cache and helper ids.
-template <int METRIC, int STORE, int SHAPE>
+template <int METRIC, int STORE, int SHAPE, bool BG = false>
 __global__ __launch_bounds__(128) void search_duo(SearchParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *duo = smem + P.wave_lds;
@@ -1352,7 +1384,7 @@ __global__ __launch_bounds__(128) void search_duo(SearchParams P) {
     // both waves of a prioritised tier: the helper's distances are on the search wave's path
     if (P.wave_prio) __builtin_amdgcn_s_setprio(2);
     if (threadIdx.x >= 64) {
-        duo_helper<METRIC, STORE, SHAPE>(P, smem, duo);
+        duo_helper<METRIC, STORE, SHAPE, BG>(P, smem, duo);
         return;
     }
     const uint32_t count = *P.work_count;
@@ -1362,7 +1394,7 @@ __global__ __launch_bounds__(128) void search_duo(SearchParams P) {
         const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
         if (t >= count) break;
         const int qi = P.qidx ? (int)P.qidx[t] : (int)t;
-        search_query<METRIC, false, STORE, SHAPE, false, true>(P, qi, (int)blockIdx.x, smem, duo);
+        search_query<METRIC, false, STORE, SHAPE, false, true, BG>(P, qi, (int)blockIdx.x, smem, duo);
     }
     lds_fence();
     if (lane_id() == 0) lds_st(&reinterpret_cast<DuoMail *>(duo)->done, 1u);

@@ -2,8 +2,8 @@
 #include "search_inst.hpp"
 
 namespace song {
-SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent, bool bagl) {
-    return pick_kernel<M_L2>(exact, store, shape, persistent, bagl);
+SearchFn search_fn_l2(bool exact, int store, int shape, bool persistent, bool bagl, bool bg) {
+    return pick_kernel<M_L2>(exact, store, shape, persistent, bagl, bg);
 }
-SearchFn duo_fn_l2(int shape) { return pick_duo<M_L2>(shape); }
+SearchFn duo_fn_l2(int shape, bool bg) { return pick_duo<M_L2>(shape, bg); }
 }  // namespace song

@@ -86,6 +86,7 @@ extern "C" void song_search_opts_default(song_search_opts *o) {
     o->rerank = 1;
     o->visited_bits = 0;
     o->bloom_k = 0;
+    o->best_in_global = -1;
 }
 
 // Low-precision copy of one row per thread: BF16 with a power-of-two scale (round to
@@ -232,8 +233,10 @@ extern "C" int song_index_create(const float *data, uint64_t n, int dim, const u
     };
     if (hipGetDevice(&ix->device) != hipSuccess) return cleanup(fail(SONG_E_HIP, "hipGetDevice failed"));
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ix->device) == hipSuccess)
+    if (hipGetDeviceProperties(&prop, ix->device) == hipSuccess) {
         ix->max_lds = std::max<size_t>(65536, prop.sharedMemPerBlock);
+        ix->n_cu = std::max(1, prop.multiProcessorCount);
+    }
 
     size_t dbytes = (size_t)n * ix->ld * sizeof(float);
     if (hipMalloc(&ix->d_data, dbytes) != hipSuccess) return cleanup(fail(SONG_E_HIP, "hipMalloc data %zu", dbytes));
@@ -307,12 +310,13 @@ extern "C" int song_index_integer_valued(const song_index *ix) { return ix ? ix-
 // ---------------------------------------------------------------- launch planning
 
 // must match the per-wave layout of search_query: visited (2^hlog slots of 4 or, compact, 2
-// bytes) | head merge scratch (64 keys) | best (pq rounded to even) | 256 candidate ids / radix
-// bins | query (the head is in registers, the bag in global memory)
-size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k, int bag_lds) {
+// bytes) | head merge scratch (64 keys) | best (pq rounded to even; not with bglob) | 256
+// candidate ids / radix bins | query (the head is in registers, the bag in global memory)
+size_t song::lds_bytes(bool exact, int pq, int qcap, int hlog, int qld, int vis16, int bloom_k, int bag_lds,
+                       int bglob) {
     size_t key = exact ? 16 : 8;
     const size_t vis = bloom_k > 0 ? ((size_t)4 << hlog) : ((size_t)(vis16 ? 2 : 4) << hlog);
-    return vis + (size_t)64 * key + (size_t)((pq + 1) & ~1) * key + 256 * 4 + (size_t)qld * 4 +
+    return vis + (size_t)64 * key + (bglob ? 0 : (size_t)((pq + 1) & ~1) * key) + 256 * 4 + (size_t)qld * 4 +
            (bag_lds ? (size_t)qcap * key : 0);
 }
 
@@ -343,7 +347,11 @@ extern "C" int song_bloom_size(int pq, int *hash_log2, int *bloom_k) {
     return SONG_OK;
 }
 
-size_t song::bag_stride(const Plan &p) { return ((size_t)p.qcap * (p.exact ? 16 : 8) + 255) & ~(size_t)255; }
+size_t song::bag_stride(const Plan &p) {
+    const size_t key = p.exact ? 16 : 8;
+    const size_t bag = ((size_t)p.qcap * key + 255) & ~(size_t)255;  // as bag_bytes in the kernel
+    return p.bglob ? bag + (((size_t)p.pq * key + 255) & ~(size_t)255) : bag;
+}
 
 size_t song::group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, int vis16, int lanes) {
     const size_t key = exact ? 16 : 8;
@@ -353,7 +361,8 @@ size_t song::group_lds_bytes(bool exact, int pq, int slack, int hlog, int qld, i
 }
 
 int song::make_plan(const song_index *ix, bool exact, int store, int pq, int slack, int hlog, Plan *p, int shape,
-                    int visited_bits, int bloom_k, int queries_per_wave, int bag_lds, int waves_per_query) {
+                    int visited_bits, int bloom_k, int queries_per_wave, int bag_lds, int waves_per_query,
+                    int best_in_global, uint64_t nq_hint) {
     if (pq > 4096) return fail(SONG_E_ARG, "pq=%d exceeds 4096", pq);
     if (queries_per_wave != 1 && queries_per_wave != 2 && queries_per_wave != 4)
         return fail(SONG_E_ARG, "queries_per_wave=%d not 1, 2 or 4", queries_per_wave);
@@ -371,6 +380,12 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
         return fail(SONG_E_ARG, "two waves per query need FP32 mode on float32 rows, the exact visited table, one "
                                 "query per wave and the bag in global memory");
     p->duo = waves_per_query == 2 ? 1 : 0;
+    if (best_in_global < -1 || best_in_global > 1) return fail(SONG_E_ARG, "best_in_global=%d", best_in_global);
+    const bool bglob_ok = queries_per_wave == 1 && !bag_lds && store == SONG_STORE_F32;
+    if (best_in_global == 1 && !bglob_ok)
+        return fail(SONG_E_ARG, "the best list in global memory needs one query per wave, float32 rows and the bag "
+                                "in global memory");
+    p->pq = pq;
     if (bloom_k < -1 || bloom_k > 9) return fail(SONG_E_ARG, "bloom_k=%d not in [-1,9]", bloom_k);
     if (bloom_k != 0) {
         int bh = 0, bk = 0;
@@ -393,11 +408,17 @@ int song::make_plan(const song_index *ix, bool exact, int store, int pq, int sla
     // compact visited slots when an id's bits beyond the group index (groups of 8) number 0..13
     const int lob = ix->idbits - (hlog - 3);
     p->vis16 = (bloom_k == 0 && visited_bits != 32 && hlog >= 4 && lob >= 0 && lob <= 13) ? 1 : 0;
+    p->bglob = best_in_global == 1 ? 1 : 0;
+    if (best_in_global < 0 && bglob_ok) {
+        const size_t lds_front = (lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16, bloom_k, 0, 0) + 15) & ~15;
+        const uint64_t per_cu = std::max<uint64_t>(1, ix->max_lds / lds_front);
+        p->bglob = (nq_hint == 0 || nq_hint > per_cu * (uint64_t)ix->n_cu) ? 1 : 0;
+    }
     if (p->lanes < 64) {
         p->shape = std::min(p->shape, 3);  // a row's lanes must lie in one sub-group
         p->lds = group_lds_bytes(p->exact, pq, slack, hlog, p->qld, p->vis16, p->lanes);
     } else {
-        p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16, bloom_k, p->bag_lds);
+        p->lds = lds_bytes(p->exact, pq, p->qcap, hlog, p->qld, p->vis16, bloom_k, p->bag_lds, p->bglob);
     }
     if (p->lds > ix->max_lds)
         return fail(SONG_E_OVERFLOW, "search needs %zu bytes of LDS per query (> %zu)", p->lds, ix->max_lds);
@@ -414,9 +435,9 @@ int song::default_hlog(int pq) { return pq > 1024 ? 13 : 12; }
 SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
     if (p.duo && persistent) {
         switch (metric) {
-        case SONG_L2: return duo_fn_l2(p.shape);
-        case SONG_IP: return duo_fn_ip(p.shape);
-        default: return duo_fn_cos(p.shape);
+        case SONG_L2: return duo_fn_l2(p.shape, p.bglob);
+        case SONG_IP: return duo_fn_ip(p.shape, p.bglob);
+        default: return duo_fn_cos(p.shape, p.bglob);
         }
     }
     if (p.lanes < 64) {
@@ -428,9 +449,9 @@ SearchFn song::search_fn(int metric, const Plan &p, bool persistent) {
     }
     const bool bagl = p.bag_lds && persistent;
     switch (metric) {
-    case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape, persistent, bagl);
-    case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape, persistent, bagl);
-    default: return search_fn_cos(p.exact, p.store, p.shape, persistent, bagl);
+    case SONG_L2: return search_fn_l2(p.exact, p.store, p.shape, persistent, bagl, p.bglob);
+    case SONG_IP: return search_fn_ip(p.exact, p.store, p.shape, persistent, bagl, p.bglob);
+    default: return search_fn_cos(p.exact, p.store, p.shape, persistent, bagl, p.bglob);
     }
 }
 
@@ -537,7 +558,8 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
     hlog = std::max(8, std::min(hlog, 16));
     if (opts.bloom_k != 0) hlog = opts.hash_log2 > 0 ? std::min(opts.hash_log2, 15) : 0;  // 0: sized for pq
     Plan plan;
-    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan, -1, opts.visited_bits, opts.bloom_k);
+    int rc = make_plan(ix, exact, opts.storage, pq, opts.slack, hlog, &plan, -1, opts.visited_bits, opts.bloom_k, 1, 0,
+                       1, opts.best_in_global, nrun);
     if (rc != SONG_OK) return rc;
 
     uint32_t *flags = d_flags;
@@ -611,7 +633,7 @@ extern "C" int song_search_device_ex(song_index *ix, const float *d_queries, uin
             int h2 = vis ? plan.hlog + 2 : plan.hlog;
             int s2 = que ? plan.slack * 4 + 64 : plan.slack;
             rc = make_plan(ix, exact, opts.storage, pq, s2, std::min(h2, plan.bloom_k ? 15 : 16), &p2, -1,
-                           opts.visited_bits, plan.bloom_k);
+                           opts.visited_bits, plan.bloom_k, 1, 0, 1, opts.best_in_global, redo.size());
             if (rc != SONG_OK) break;
             plan = p2;
             if (!d_idx && hipMalloc(&d_idx, nq * sizeof(uint32_t)) != hipSuccess) {

@@ -476,6 +476,47 @@ def test_two_waves_per_query_equal_plain_and_reference(pq):
         b.close()
 
 
+@pytest.mark.parametrize("mode", ["exact", "fp32"])
+def test_best_list_in_global_memory_equals_plain(mode):
+    """Tiers with the best list's front in each wave slot's global scratch (best_in_global 1) or in
+    LDS (0): ids, distances and flags equal the plain search's at a budget where the front holds
+    most of the list; with small tables and no slack (spills, overflow re-runs) and, in FP32
+    mode, with two waves per query; eager runs and graph replays."""
+    import torch
+
+    meta, base, ix, q, ref_ids = _mix20k()
+    m = song_amd.MODE_EXACT if mode == "exact" else song_amd.MODE_FP32
+    nq = q.shape[0]
+    pq, k = 500, 100
+    k_h = np.full(nq, k, dtype=np.int32)
+    pq_h = np.full(nq, pq, dtype=np.int32)
+    want = _plain(ix, q, k_h, pq_h, m)
+    variants = [[tier(pq, best_in_global=1)], [tier(pq, best_in_global=1, hash_log2=9, slack=0, groups_per_cu=1)],
+                [tier(100, best_in_global=1), tier(pq, best_in_global=1, waves_per_group=2)],
+                [tier(pq, best_in_global=0)], [tier(pq, best_in_global=0, hash_log2=9, slack=0, groups_per_cu=1)]]
+    if m == song_amd.MODE_FP32:
+        variants += [[tier(pq, best_in_global=1, waves_per_query=2)], [tier(pq, best_in_global=0, waves_per_query=2)]]
+    kd = torch.from_numpy(k_h).to(q.device)
+    pd = torch.from_numpy(pq_h).to(q.device)
+    for tiers in variants:
+        print(f"[best in global] {mode} {tiers}", flush=True)
+        b = song_amd.Batch(ix, tiers, nq, mode=m)
+        for graph in (False, True):
+            ids = torch.full((nq, k), -7, dtype=torch.int32, device=q.device)
+            ds = torch.zeros((nq, k), dtype=torch.float32, device=q.device)
+            fl = torch.full((nq,), -1, dtype=torch.int32, device=q.device)
+            if graph:
+                side = torch.cuda.Stream()
+                side.wait_stream(torch.cuda.current_stream())
+                for _ in range(2):
+                    b.run(q, ids, fl, pq_arr=pd, k_arr=kd, dists=ds, graph=True, stream=side.cuda_stream)
+            else:
+                b.run(q, ids, fl, pq_arr=pd, k_arr=kd, dists=ds)
+            torch.cuda.synchronize()
+            _same((ids, ds, fl), want, f"best in global {mode} {tiers} graph={graph}")
+        b.close()
+
+
 def test_two_waves_per_query_on_inner_product(ip96):
     c = ip96
     want = _plain(c["ix"], c["q"], c["k"], c["pq"], song_amd.MODE_FP32)

@@ -103,3 +103,55 @@ def test_visited_slot_widths_give_the_reference_ids(hash_log2):
         ids, ds, flags = ix.search(query, meta["pq"], mode=song_amd.MODE_EXACT, hash_log2=hash_log2,
                                    visited_bits=bits)
         assert np.array_equal(ids, ref_ids), bits
+
+
+@pytest.mark.parametrize("mode", ["exact", "fp32"])
+def test_best_list_front_in_global_memory_matches_oracle(mode):
+    """The pq-best list's front (all but its last 64 entries) in LDS (best_in_global 0) and in the
+    query slot's global scratch (1, the default where allowed): at pq 500 the front holds most of
+    the list, and every insertion path (growth, an expansion landing in the front, the full list's
+    front shift) runs.  Full pq lists, distances and counters equal the oracle's in both; the
+    golden ids at the golden pq too, and the no-allocation entry point agrees."""
+    import torch
+
+    meta, base, query, ref_ids, graph = golden_cases.load("mix_l2_20k")
+    graph = _graph(meta, base, graph)
+    ix = song_amd.Index(base, graph, meta["metric"])
+    m = song_amd.MODE_EXACT if mode == "exact" else song_amd.MODE_FP32
+    print(f"[best in global] mix_l2_20k {mode}", flush=True)
+    ids_g, _, flags = ix.search(query, meta["pq"], mode=m, best_in_global=1)
+    assert np.array_equal(ids_g, ref_ids)
+    pq = 500
+    o_ids, o_ds, o_exp, o_vis = pyoracle.search_batch(base, graph, meta["metric"], query, pq, threads=8,
+                                                      with_stats=True)
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(query).to(dev)
+    nq = q.shape[0]
+    outs = []
+    for bg in (0, 1):
+        ids = torch.full((nq, pq), -7, dtype=torch.int32, device=dev)
+        ds = torch.empty((nq, pq), dtype=torch.float32, device=dev)
+        fl = torch.empty(nq, dtype=torch.int32, device=dev)
+        st = torch.empty((nq, 8), dtype=torch.int32, device=dev)
+        ix.search_device(q, pq, pq, ids, ds, fl, st, mode=m, best_in_global=bg)
+        torch.cuda.synchronize()
+        g_ids = ids.cpu().numpy().astype(np.int64)
+        o = o_ids.astype(np.int64)
+        o[o_ids == np.uint64(0xFFFFFFFFFFFFFFFF)] = -1
+        assert np.array_equal(g_ids, o), f"best_in_global={bg}"
+        valid = o >= 0
+        ref = o_ds[valid]
+        err = np.abs(ds.cpu().numpy().astype(np.float64)[valid] - ref) / np.maximum(np.abs(ref), 1e-30)
+        assert err.max() <= DIST_RTOL
+        st = st.cpu().numpy()
+        assert np.array_equal(st[:, 0], o_exp) and np.array_equal(st[:, 1], o_vis)
+        outs.append((ids, ds, fl))
+    # the no-allocation entry point with the front in its static scratch
+    ids = torch.full((nq, pq), -7, dtype=torch.int32, device=dev)
+    ds = torch.empty((nq, pq), dtype=torch.float32, device=dev)
+    fl = torch.empty(nq, dtype=torch.int32, device=dev)
+    song_amd.search_device_noalloc(ix, q, pq, pq, ids, ds, fl, mode=m, best_in_global=1)
+    torch.cuda.synchronize()
+    ok = (fl.cpu().numpy() & (song_amd.FLAG_VISITED_OVF | song_amd.FLAG_QUEUE_OVF)) == 0
+    assert ok.sum() > 0.9 * nq
+    assert torch.equal(ids[torch.from_numpy(ok).to(dev)], outs[1][0][torch.from_numpy(ok).to(dev)])

@@ -97,9 +97,9 @@ def test_persistent_loop_is_wave_uniform(search_asm):
     kernels = {}
     for text in search_asm:
         kernels.update(_functions(text, r"_ZN4song17search_persistent"))
-    # 3 metrics x (exact, fp32) x 3 row storages x 5 row shapes, and the LDS-bag form
-    # (float32 rows) of each metric, mode and shape
-    assert len(kernels) == 90
+    # 3 metrics x (exact, fp32) x 3 row storages x 5 row shapes, and the LDS-bag and global
+    # best-list forms (float32 rows) of each metric, mode and shape
+    assert len(kernels) == 120
     bad = [k for k, body in kernels.items() if not loop_head_is_uniform(body)]
     assert not bad, f"{len(bad)} persistent kernels with a split work loop, e.g. {bad[:3]}"
 
@@ -108,7 +108,7 @@ def test_two_wave_kernel_work_loop_is_wave_uniform(search_asm):
     kernels = {}
     for text in search_asm:
         kernels.update(_functions(text, r"_ZN4song10search_duo"))
-    assert len(kernels) == 15  # 3 metrics x 5 row shapes (FP32 mode, float32 rows)
+    assert len(kernels) == 30  # 3 metrics x 5 row shapes x best list front in LDS / global (FP32 mode)
     bad = [k for k, body in kernels.items() if not work_loop_is_uniform(body)]
     assert not bad, f"{len(bad)} two-wave kernels with a split work loop, e.g. {bad[:3]}"
 
