@@ -6,7 +6,7 @@
 //   B[k][n]: n = l % 16, k = 4*(l/16) + j
 //   D[i][n]: n = l % 16, i = 4*(l/16) + j
 // Training runs as separate kernels (GEMM with bias/ReLU/mask epilogues, softmax
-// cross-entropy, column sums, Adam); inference is one fused kernel per 64 queries.
+// cross-entropy, column sums, Adam); inference is one fused kernel, 16 queries per workgroup.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -263,14 +263,14 @@ __global__ __launch_bounds__(64) void features_kernel(const float *Q, int dim, c
 
 // ---------------------------------------------------------------- fused inference
 
-// queries per workgroup: 32 rows (MI = 2 row tiles of 16) keep the 4 waves' LDS to ~45 KB, so
-// a 10k-query batch launches 313 workgroups, two or three per CU (64 rows held the batch to 157
-// CUs at one wave per SIMD, and the output layer waited on its weight loads)
-constexpr int IQ = 32;
+// queries per workgroup: 16 rows (MI = 1 row tile) -- a 10k-query batch launches 625 workgroups,
+// two or three per CU, and each wave's chain of MFMAs and softmax updates is half that of 32 rows;
+// the output weights are read twice as often, from L2.  For 10k queries: 16 rows 197 us, 32 rows
+// 256 us, 64 rows 447 us (157 workgroups, one wave per SIMD)
+constexpr int IQ = 16;
 constexpr int MI = IQ / 16;
 // waves per workgroup: they split the output layer's columns (64 per wave and chunk) and the
-// hidden units (slices of 64); 8 waves per group, each with half the chain, measured slower (279
-// against 256 us for 10k queries)
+// hidden units (slices of 64); with 16 rows, 2 waves per group measured 319 us and 8 waves 206 us
 constexpr int IW = 4;
 constexpr int IT = 64 * IW;  // threads per workgroup
 
