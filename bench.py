@@ -715,6 +715,10 @@ def bench_c1(args, torch, song_amd):
 
     dev = torch.device("cuda:0")
     base, query, graph, metric, _ = workloads.load("c1_15k")
+    if args.c1_repeat > 1:
+        # the 5,000 queries repeated: the same per-query work at a batch that no longer fits the
+        # device at once (throughput-bound instead of latency-bound)
+        query = np.ascontiguousarray(np.tile(query, (args.c1_repeat, 1)))
     ix = song_amd.Index(base, graph, metric)
     nq, pq, k = query.shape[0], args.c1_pq, args.k
     q = torch.from_numpy(query).to(dev)
@@ -762,7 +766,9 @@ def bench_c1(args, torch, song_amd):
         "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 products, fp64 sums (exact)", "data": "synthetic: seeded uniform rows, graph = reference insertion build",
-        "config": {"workload": f"15,000 x 16 uniform [-1,1] cosine, {nq:,} queries, pq {pq}, top-{k}",
+        "config": {"workload": (f"15,000 x 16 uniform [-1,1] cosine, {nq:,} queries"
+                                + (f" (5,000 repeated {args.c1_repeat}x)" if args.c1_repeat > 1 else "")
+                                + f", pq {pq}, top-{k}"),
                    "plain": {"qps": round(nq / (ms_plain / 1e3), 1), "ms_per_step": round(ms_plain, 4),
                              "recall@10": round(r, 4)},
                    "batched": res},
@@ -874,6 +880,8 @@ def main():
     ap.add_argument("--c4-nq", type=int, default=100_000)
     ap.add_argument("--c4-recall-sample", type=int, default=10_000)
     ap.add_argument("--c1-pq", type=int, default=100)
+    ap.add_argument("--c1-repeat", type=int, default=1,
+                    help="C1: repeat the 5,000 queries this many times (a throughput-bound batch)")
     ap.add_argument("--c5-n", type=int, default=10_000_000)
     ap.add_argument("--c5-nq", type=int, default=100_000)
     ap.add_argument("--c3-n", type=int, default=1_000_000)
