@@ -14,7 +14,7 @@
  *        exact argument list (h_data, h_graph, shift, num, dim) for a one-shot
  *        call (uploads, searches, frees).
  *   KernelFixedDegreeGraph::search_top_k_batch (kernelgraph.h:269-272) calls
- *        either; see INTEGRATION.md for the host-side binding.
+ *        either; song_amd/cli/kernelgraph.hpp is that binding written out.
  */
 #ifndef SONG_AMD_H
 #define SONG_AMD_H
