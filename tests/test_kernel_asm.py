@@ -123,3 +123,30 @@ def test_checker_flags_a_split_loop():
     assert work_loop_is_uniform(["s_cbranch_scc1 .LBB0_9", ".LBB0_2:  ; Inner Loop Header: Depth=1", "s_sleep 2",
                                  "s_branch .LBB0_2"] + good)
     assert not work_loop_is_uniform(["s_cbranch_scc1 .LBB0_9"] + split)
+
+
+def _kernel_metadata(text):
+    """{kernel name: {field: int}} from the code object metadata of an assembly listing."""
+    meta = text[text.index("amdhsa.kernels"):]
+    out = {}
+    for block in re.split(r"\n  - \.", meta):
+        m = re.search(r"\.name:\s+(\S+)", block)
+        if not m:
+            continue
+        fields = {k: int(v) for k, v in re.findall(r"\.(vgpr_spill_count|private_segment_fixed_size|vgpr_count):\s+(\d+)",
+                                                   block)}
+        out[m.group(1)] = fields
+    return out
+
+
+def test_search_kernels_keep_every_vector_register_in_registers(search_asm):
+    """No search kernel spills vector registers to scratch memory.  The kernels keep per-lane state
+    (the queue head, the best list's tail) live across divergent regions; a spill under a partial
+    exec mask would save only the active lanes' values, and scratch round trips would also lengthen
+    every expansion."""
+    kernels = {}
+    for text in search_asm:
+        kernels.update({k: v for k, v in _kernel_metadata(text).items() if "search" in k})
+    assert len(kernels) == 3 * 80
+    bad = {k: v for k, v in kernels.items() if v.get("vgpr_spill_count", 0) or v.get("private_segment_fixed_size", 0)}
+    assert not bad, f"{len(bad)} search kernels use scratch, e.g. {list(bad.items())[:2]}"
