@@ -83,6 +83,17 @@ def test_auto_mode_on_integer_data_matches(case):
         assert np.array_equal(ids, ref_ids)
 
 
+def test_best_list_front_placement_keeps_reference_ids(case):
+    """Every golden case (L2, inner product, cosine; ties), with the best list's front in LDS and
+    in global memory: the reference's ids either way."""
+    name, meta, base, query, ref_ids, graph = case
+    ix = song_amd.Index(base, graph, meta["metric"])
+    for bg in (0, 1):
+        ids, _, flags = ix.search(query, meta["pq"], mode=song_amd.MODE_EXACT, best_in_global=bg)
+        bad = np.nonzero((ids != ref_ids).any(1))[0]
+        assert len(bad) == 0, f"{name} best_in_global={bg}: {len(bad)} queries differ, first {bad[:5]}"
+
+
 def test_small_tables_force_spill_and_retry_and_stay_exact():
     meta, base, query, ref_ids, graph = golden_cases.load("tie_l2_3k")
     ix = song_amd.Index(base, graph, meta["metric"])
