@@ -535,6 +535,27 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
         table["step_grid_trials"] = grid_log
         log(f"learned tier grid on the whole step (groups per CU, ms): {grid_log} -> "
             f"{best_t[0]['groups_per_cu']}")
+        # the fallback tier's visited-table width and heap depth, also on the whole step: the tuner
+        # times the tier alone, and beside the learned tier's waves the two choices rank
+        # differently from run to run (results are the same either way)
+        fb_log = []
+        for key, choices in (("visited_bits", (0, 32)), ("slack", (32, 128))):
+            for v in choices:
+                if best_t[1].get(key) == v:
+                    continue
+                cand = [dict(t) for t in best_t]
+                cand[1][key] = v
+                try:
+                    ms = step_ms(cand)
+                except song_amd.SongError:
+                    continue
+                fb_log.append((key, v, round(ms, 3)))
+                if ms < best_ms * 0.99:
+                    best_t, best_ms = cand, ms
+        table["tiers"] = best_t
+        table["step_fallback_trials"] = fb_log
+        log(f"fallback tier on the whole step (knob, value, ms): {fb_log} -> visited_bits "
+            f"{best_t[1].get('visited_bits')}, slack {best_t[1].get('slack')}")
 
     def measure_test(cap):
         lb = make(nq, lpq, scale, table["tiers"], cap)
@@ -594,7 +615,8 @@ def bench_learned(args, torch, song_amd, ix, base, q, query, qs, selq, gt_d, gs_
     fit_out = {kk: vv for kk, vv in fit.items() if kk != "loss"}
     fit_out["final_loss"] = fit["loss"][-1]
     info = dict(head, pq=lpq, step_scale=ent["step_scale"], chosen_on="selection split", selection=ent,
-                tiers=table["tiers"], step_grid_trials=table.get("step_grid_trials"), capped_fallback=capped,
+                tiers=table["tiers"], step_grid_trials=table.get("step_grid_trials"),
+                step_fallback_trials=table.get("step_fallback_trials"), capped_fallback=capped,
                 training=fit_out, sweep_selection=lsweep)
     return info
 
